@@ -1,0 +1,217 @@
+"""ctypes view of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py import this
+module, and only as the checker or the CPU baseline.  The product package ``saras_amd`` never
+imports it.
+
+Arrays follow the reference layout: padded (Nx+2, Ny+2, Nz+2) float64, C order, k fastest,
+element [0] along an axis is logical index -1.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "_build")
+
+SMOOTH_LEX, SMOOTH_RB, SMOOTH_JACOBI, SMOOTH_JACOBI_RB = 0, 1, 2, 3
+SEM_REFERENCE, SEM_PROJECTED = 0, 1
+P, R, T, S = 0, 1, 2, 3  # pressureData, residualData, tmpDataArray, smoothedPres
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("xInd", C.c_int), ("yInd", C.c_int), ("zInd", C.c_int),
+        ("vcDepth", C.c_int), ("vcCount", C.c_int),
+        ("preSmooth", C.c_int), ("postSmooth", C.c_int),
+        ("resType", C.c_int), ("smoother", C.c_int),
+        ("xPer", C.c_int), ("yPer", C.c_int), ("zPer", C.c_int),
+        ("testPoisson", C.c_int), ("semantics", C.c_int),
+        ("mgTolerance", C.c_double), ("relTolerance", C.c_double),
+        ("dXi", C.c_double), ("dEt", C.c_double), ("dZt", C.c_double),
+        ("padMode", C.c_int), ("coarseProject", C.c_int),
+    ]
+
+
+def build():
+    """Compile the oracle libraries (gcc, a few seconds)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_libs = {}
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(_BUILD, name)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.or_grid_staggr.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, dp, dp]
+    L.or_grid_colloc.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, dp, dp]
+    L.or_mg_create.restype = vp
+    L.or_mg_create.argtypes = [C.POINTER(Params)] + [dp] * 12
+    L.or_mg_destroy.argtypes = [vp]
+    L.or_mg_array.restype = dp
+    L.or_mg_array.argtypes = [vp, C.c_int, C.c_int]
+    L.or_mg_dims.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    for f in ("or_mg_set_level", "or_mg_set_zero_bc", "or_mg_smooth", "or_mg_update_pads"):
+        getattr(L, f).argtypes = [vp, C.c_int]
+    L.or_mg_get_level.restype = C.c_int
+    L.or_mg_get_level.argtypes = [vp]
+    for f in ("or_mg_compute_residual", "or_mg_solve_coarsest", "or_mg_coarsen", "or_mg_prolong",
+              "or_mg_impose_bc", "or_mg_vcycle"):
+        getattr(L, f).argtypes = [vp]
+    L.or_mg_compute_error.restype = C.c_double
+    L.or_mg_compute_error.argtypes = [vp, C.c_int]
+    L.or_mg_solve.restype = C.c_int
+    L.or_mg_solve.argtypes = [vp, dp, dp, dp, C.c_int]
+    L.or_null_vector_1d.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp, dp]
+    L.or_mg_project_rhs.argtypes = [vp, dp]
+    L.or_mg_metric.restype = dp
+    L.or_mg_metric.argtypes = [vp, C.c_int, C.c_int]
+    L.or_mg_coef.restype = C.c_double
+    L.or_mg_coef.argtypes = [vp, C.c_int, C.c_int]
+    L.or_mg_max_count.restype = C.c_int
+    L.or_mg_max_count.argtypes = [vp]
+    _libs[name] = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def grid_axis(n, length=1.0, grid_type=1, beta=1.0, kind="staggr", fast=False):
+    """Node (staggr, padded n+2) or face (colloc, padded n+1) coordinates and metrics."""
+    L = lib(fast)
+    m = n + 2 if kind == "staggr" else n + 1
+    x, xi_x, xixx, xix2 = (np.zeros(m) for _ in range(4))
+    fn = L.or_grid_staggr if kind == "staggr" else L.or_grid_colloc
+    fn(n, float(length), int(grid_type), float(beta), _ptr(x), _ptr(xi_x), _ptr(xixx), _ptr(xix2))
+    return {"x": x, "xi_x": xi_x, "xixx": xixx, "xix2": xix2}
+
+
+class OracleMG:
+    """The restated multigrid_d3 object: level hierarchies plus its protected operators."""
+
+    def __init__(self, params, metrics, walls=None, fast=False):
+        self.L = lib(fast)
+        self.params = params
+        self._keep = []
+        ms = []
+        for key in ("xix2", "xixx", "ety2", "etyy", "ztz2", "ztzz"):
+            a = np.ascontiguousarray(metrics[key], dtype=np.float64)
+            self._keep.append(a)
+            ms.append(_ptr(a))
+        ws = []
+        for w in (walls or [None] * 6):
+            if w is None:
+                ws.append(None)
+            else:
+                a = np.ascontiguousarray(w, dtype=np.float64)
+                self._keep.append(a)
+                ws.append(_ptr(a))
+        self.h = self.L.or_mg_create(C.byref(params), *ms, *ws)
+        if not self.h:
+            raise RuntimeError("or_mg_create failed")
+
+    def close(self):
+        if self.h:
+            self.L.or_mg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dims(self, level):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self.L.or_mg_dims(self.h, level, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def array(self, which, level):
+        """numpy view (no copy) of a level array; valid until the next Jacobi swap."""
+        nx, ny, nz = self.dims(level)
+        p = self.L.or_mg_array(self.h, which, level)
+        return np.ctypeslib.as_array(p, shape=(nx + 2, ny + 2, nz + 2))
+
+    def metric(self, which, level):
+        n = self.dims(level)[which // 2]
+        return np.ctypeslib.as_array(self.L.or_mg_metric(self.h, which, level), shape=(n + 2,)).copy()
+
+    def coef(self, which, level):
+        return self.L.or_mg_coef(self.h, which, level)
+
+    @property
+    def level(self):
+        return self.L.or_mg_get_level(self.h)
+
+    @level.setter
+    def level(self, v):
+        self.L.or_mg_set_level(self.h, int(v))
+
+    def set_zero_bc(self, z):
+        self.L.or_mg_set_zero_bc(self.h, int(z))
+
+    def compute_residual(self):
+        self.L.or_mg_compute_residual(self.h)
+
+    def smooth(self, n):
+        self.L.or_mg_smooth(self.h, int(n))
+
+    def solve_coarsest(self):
+        self.L.or_mg_solve_coarsest(self.h)
+
+    def coarsen(self):
+        self.L.or_mg_coarsen(self.h)
+
+    def prolong(self):
+        self.L.or_mg_prolong(self.h)
+
+    def compute_error(self, norm):
+        return self.L.or_mg_compute_error(self.h, int(norm))
+
+    def impose_bc(self):
+        self.L.or_mg_impose_bc(self.h)
+
+    def update_pads(self, which):
+        self.L.or_mg_update_pads(self.h, int(which))
+
+    def vcycle(self):
+        self.L.or_mg_vcycle(self.h)
+
+    def max_count(self):
+        return self.L.or_mg_max_count(self.h)
+
+    def project_rhs(self, rhs):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        self.L.or_mg_project_rhs(self.h, _ptr(rhs))
+        return rhs
+
+    def solve(self, p, rhs, max_hist=256):
+        p = np.ascontiguousarray(p, dtype=np.float64).copy()
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        hist = np.zeros(max_hist)
+        n = self.L.or_mg_solve(self.h, _ptr(p), _ptr(rhs), _ptr(hist), max_hist)
+        return p, hist[:min(n, max_hist)].copy()
+
+
+def null_vector_1d(n, periodic, h, d2, d1):
+    L = lib()
+    w = np.zeros(n)
+    d2 = np.ascontiguousarray(d2, dtype=np.float64)
+    d1 = np.ascontiguousarray(d1, dtype=np.float64)
+    L.or_null_vector_1d(n, int(periodic), float(h), _ptr(d2), _ptr(d1), _ptr(w))
+    return w

@@ -1,0 +1,98 @@
+"""ctypes binding of the C ABI in include/saras_mg.h (libsaras_mg.so, gfx950).
+
+The library is loaded from saras_amd/_build; a missing library is an error (there is no CPU
+fallback for any solver entry point)."""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libsaras_mg.so")
+
+SR_OK, SR_ERR_ARG, SR_ERR_HIP, SR_ERR_STATE, SR_ERR_NOMEM, SR_ERR_NOT_CONVERGED = 0, 1, 2, 3, 4, 5
+SR_PRESSURE, SR_RESIDUAL, SR_TMP = 0, 1, 2
+SR_SEM_REFERENCE, SR_SEM_PROJECTED = 0, 1
+SR_PREC_FP64, SR_PREC_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+
+
+class SrMgDesc(C.Structure):
+    _fields_ = [
+        ("xInd", C.c_int), ("yInd", C.c_int), ("zInd", C.c_int),
+        ("vcDepth", C.c_int), ("vcCount", C.c_int),
+        ("preSmooth", C.c_int), ("postSmooth", C.c_int),
+        ("gsSmooth", C.c_int), ("resType", C.c_int),
+        ("xPer", C.c_int), ("yPer", C.c_int), ("zPer", C.c_int),
+        ("testPoisson", C.c_int), ("semantics", C.c_int), ("precision", C.c_int),
+        ("padMode", C.c_int), ("coarseProject", C.c_int),
+        ("mgTolerance", C.c_double), ("relTolerance", C.c_double),
+        ("dXi", C.c_double), ("dEt", C.c_double), ("dZt", C.c_double),
+        ("xix2", _dp), ("xixx", _dp), ("ety2", _dp), ("etyy", _dp), ("ztz2", _dp), ("ztzz", _dp),
+        ("wall", _dp * 6),
+    ]
+
+
+class SrMgStats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("converged", C.c_int), ("residual", C.c_double), ("solve_ms", C.c_double),
+                ("sweep0_ms", C.c_double)]
+
+
+# (name, restype, argtypes) of every entry point declared in include/saras_mg.h
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+SIGNATURES = [
+    ("sr_mg_create", C.c_int, [C.POINTER(SrMgDesc), C.POINTER(C.c_void_p)]),
+    ("sr_mg_destroy", C.c_int, [_vp]),
+    ("sr_mg_solve", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SrMgStats)]),
+    ("sr_mg_history", C.c_int, [_vp, _dp, C.c_int]),
+    ("sr_mg_set_level", C.c_int, [_vp, C.c_int]),
+    ("sr_mg_get_level", C.c_int, [_vp, _ip]),
+    ("sr_mg_set_zero_bc", C.c_int, [_vp, C.c_int]),
+    ("sr_mg_upload", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    ("sr_mg_download", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    ("sr_mg_level_dims", C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
+    ("sr_mg_op_smooth", C.c_int, [_vp, C.c_int]),
+    ("sr_mg_op_compute_residual", C.c_int, [_vp]),
+    ("sr_mg_op_coarsen", C.c_int, [_vp]),
+    ("sr_mg_op_prolong", C.c_int, [_vp]),
+    ("sr_mg_op_solve_coarsest", C.c_int, [_vp]),
+    ("sr_mg_op_impose_bc", C.c_int, [_vp]),
+    ("sr_mg_op_update_pads", C.c_int, [_vp, C.c_int]),
+    ("sr_mg_op_compute_error", C.c_int, [_vp, C.c_int, _dp]),
+    ("sr_mg_op_vcycle", C.c_int, [_vp]),
+    ("sr_mg_time_kernel", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    ("sr_mg_kernel_bytes", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    ("sr_grid_axis", C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+    ("sr_last_error", C.c_char_p, []),
+]
+
+_lib = None
+
+
+class SarasError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"saras_mg error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libsaras_mg.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != SR_OK:
+        msg = load().sr_last_error()
+        raise SarasError(code, msg.decode() if msg else "")
+    return code

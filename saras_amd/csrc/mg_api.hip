@@ -1,0 +1,525 @@
+// C ABI of the multigrid solver (include/saras_mg.h): level hierarchy, V-cycle orchestration,
+// hipGraph capture of the cycle, and the per-operator entry points.
+//
+// Restates the control flow of lib/poisson/poisson.cc (mgSolve 100-151, vCycle 219-280) and the
+// construction of lib/poisson/poisson_d3.cc:64-87 on the device kernels of mg_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/saras_mg.h"
+#include "mg_kernels.h"
+#include "mg_state.h"
+
+using namespace sr;
+
+// ------------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+extern "C" const char *sr_last_error(void) { return g_err.c_str(); }
+
+int sr::set_err(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess)                                                                         \
+            return set_err(SR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+#define RETCHK(expr)                                                                                  \
+    do {                                                                                              \
+        int _r = (expr);                                                                              \
+        if (_r != SR_OK) return _r;                                                                   \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------
+// construction
+// ------------------------------------------------------------------------------------------------
+static Lay make_lay(int nx, int ny, int nz) {
+    Lay L;
+    L.nx = nx;
+    L.ny = ny;
+    L.nz = nz;
+    L.kh = (nz + 1) / 2 + 1;
+    L.khs = L.kh;
+    return L;
+}
+
+// Left null vector of one direction (periodic: trapezoid weights; mirror ghosts: the balance
+// w_k c_k = w_{k+1} a_{k+1} of a zero-row-sum three-point operator).
+static std::vector<double> null_vector(int n, bool periodic, double h, const double *d2, const double *d1) {
+    std::vector<double> w(n, 1.0);
+    if (periodic) {
+        w[0] = 0.5;
+        w[n - 1] = 0.5;
+        return w;
+    }
+    double h2 = h * h;
+    w[0] = 1.0;
+    for (int k = 0; k < n - 1; k++) {
+        double ck = d2[k] / h2 + d1[k] / (2.0 * h);
+        double ak1 = d2[k + 1] / h2 - d1[k + 1] / (2.0 * h);
+        if (k == 0) ck += d2[0] / h2 - d1[0] / (2.0 * h);
+        if (k + 1 == n - 1) ak1 += d2[n - 1] / h2 + d1[n - 1] / (2.0 * h);
+        w[k + 1] = w[k] * ck / ak1;
+    }
+    return w;
+}
+
+template <typename T>
+static int upload_vec(T **dst, const std::vector<T> &v) {
+    HIPCHK(hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(v.size(), 1)));
+    HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return SR_OK;
+}
+
+static int create_impl(const sr_mg_desc *d, sr_mg *m) {
+    m->d = *d;
+    for (int q = 0; q < 6; q++) m->d.wall[q] = nullptr;
+    int ind[3] = {d->xInd, d->yInd, d->zInd};
+    int n0[3];
+    for (int a = 0; a < 3; a++) {
+        if (ind[a] < 1 || ind[a] > 14) return set_err(SR_ERR_ARG, "size index out of range 1..14");
+        n0[a] = (1 << ind[a]) + 1;
+    }
+    int depth = d->vcDepth;
+    if (depth < 1 || depth >= *std::min_element(ind, ind + 3))
+        return set_err(SR_ERR_ARG, "vcDepth must satisfy 1 <= vcDepth < min(xInd, yInd, zInd)");
+    if (d->preSmooth < 0 || d->postSmooth < 0 || d->vcCount < 0) return set_err(SR_ERR_ARG, "negative counts");
+    const double *met[6] = {d->xix2, d->xixx, d->ety2, d->etyy, d->ztz2, d->ztzz};
+    for (int q = 0; q < 6; q++)
+        if (!met[q]) return set_err(SR_ERR_ARG, "metric array missing");
+    for (int q = 0; q < 6; q++) m->met0[q].assign(met[q], met[q] + n0[q / 2] + 2);
+
+    HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&m->ev0));
+    HIPCHK(hipEventCreate(&m->ev1));
+    HIPCHK(hipEventCreate(&m->sw0));
+    HIPCHK(hipEventCreate(&m->sw1));
+    m->depth = depth;
+    m->lv.resize(depth + 1);
+    double dxi[3] = {d->dXi, d->dEt, d->dZt};
+    for (int l = 0; l <= depth; l++) {
+        Level &V = m->lv[l];
+        V.L = make_lay((1 << (ind[0] - l)) + 1, (1 << (ind[1] - l)) + 1, (1 << (ind[2] - l)) + 1);
+        int nl[3] = {V.L.nx, V.L.ny, V.L.nz};
+        double s = (double)(1 << l);
+        for (int a = 0; a < 3; a++) {
+            double h = s * dxi[a];   // hx(l) = strideValues(l) * dXi (poisson.cc:420-424)
+            V.h[a] = h;
+            // coarse metrics = fine metrics sampled at stride 2^l, pads 0 (poisson.cc:467-494)
+            std::vector<double> c(4 * (size_t)(nl[a] + 2), 0.0);
+            for (int i = 0; i < nl[a]; i++) {
+                double m2 = met[2 * a][i * (1 << l) + 1], m1 = met[2 * a + 1][i * (1 << l) + 1];
+                c[4 * (i + 1) + 0] = m2 / (h * h) + m1 / (2.0 * h);
+                c[4 * (i + 1) + 1] = m2 / (h * h) - m1 / (2.0 * h);
+                c[4 * (i + 1) + 2] = 2.0 * m2 / (h * h);
+            }
+            V.coef_h[a] = c;
+            RETCHK(upload_vec(&V.c64[a], c));
+            std::vector<float> cf(c.begin(), c.end());
+            RETCHK(upload_vec(&V.c32[a], cf));
+        }
+        long sz = lay_size(V.L);
+        HIPCHK(hipMalloc((void **)&V.u, sizeof(double) * sz));
+        HIPCHK(hipMalloc((void **)&V.f, sizeof(double) * sz));
+        HIPCHK(hipMalloc((void **)&V.t, sizeof(double) * sz));
+        launch_fill<double>(V.u, sz, 0.0, m->stream);
+        launch_fill<double>(V.f, sz, 0.0, m->stream);
+        launch_fill<double>(V.t, sz, 0.0, m->stream);
+    }
+    const Lay &C = m->lv[depth].L;
+    m->maxCount = (C.nx + 1) * (C.ny + 1) * (C.nz + 1);   // poisson.cc:359-361
+    if (coarsest_lds_bytes<double>(C) > 60 * 1024)
+        return set_err(SR_ERR_ARG, "coarsest level too large for the single-workgroup solve; increase vcDepth");
+
+    // Dirichlet wall planes (TEST_POISSON), level 0 only
+    const Lay &F = m->lv[0].L;
+    long wsz[3] = {(long)(F.ny + 2) * (F.nz + 2), (long)(F.nx + 2) * (F.nz + 2), (long)(F.nx + 2) * (F.ny + 2)};
+    for (int q = 0; q < 6; q++) {
+        if (d->wall[q]) {
+            std::vector<double> w(d->wall[q], d->wall[q] + wsz[q / 2]);
+            RETCHK(upload_vec(&m->wall[q], w));
+        }
+    }
+    // null vectors for the range projection (finest level)
+    const Level &V0 = m->lv[0];
+    int per[3] = {d->xPer, d->yPer, d->zPer};
+    double wsum[3];
+    for (int a = 0; a < 3; a++) {
+        int n = n0[a];
+        auto w = null_vector(n, per[a] != 0, V0.h[a], met[2 * a] + 1, met[2 * a + 1] + 1);
+        wsum[a] = 0;
+        for (double x : w) wsum[a] += x;
+        RETCHK(upload_vec(&m->wvec[a], w));
+    }
+    m->wsum = wsum[0] * wsum[1] * wsum[2];
+    // null vectors of the coarsest level, from its sampled metrics (coarseProject)
+    {
+        Level &Cv = m->lv[depth];
+        int nc[3] = {Cv.L.nx, Cv.L.ny, Cv.L.nz};
+        double prod = 1.0;
+        for (int a = 0; a < 3; a++) {
+            std::vector<double> d2(nc[a]), d1(nc[a]);
+            for (int i = 0; i < nc[a]; i++) {
+                d2[i] = met[2 * a][i * (1 << depth) + 1];
+                d1[i] = met[2 * a + 1][i * (1 << depth) + 1];
+            }
+            auto w = null_vector(nc[a], per[a] != 0, Cv.h[a], d2.data(), d1.data());
+            double sw = 0;
+            for (double x : w) sw += x;
+            prod *= sw;
+            RETCHK(upload_vec(&Cv.wc[a], w));
+        }
+        Cv.wcsum = prod;
+    }
+
+    int nparts = norm_parts(F);
+    HIPCHK(hipMalloc((void **)&m->part, sizeof(double) * 4 * (size_t)nparts));
+    HIPCHK(hipMalloc((void **)&m->dres, sizeof(double) * 16));
+    HIPCHK(hipMalloc((void **)&m->diters, sizeof(int) * 4));
+    HIPCHK(hipHostMalloc((void **)&m->hres, sizeof(double) * 16, hipHostMallocDefault));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return SR_OK;
+}
+
+extern "C" int sr_mg_create(const sr_mg_desc *desc, sr_mg **out) {
+    if (!desc || !out) return set_err(SR_ERR_ARG, "null argument");
+    *out = nullptr;
+    sr_mg *m = new sr_mg();
+    int r = create_impl(desc, m);
+    if (r != SR_OK) {
+        std::string keep = g_err;
+        sr_mg_destroy(m);
+        g_err = keep;
+        return r;
+    }
+    *out = m;
+    return SR_OK;
+}
+
+extern "C" int sr_mg_destroy(sr_mg *m) {
+    if (!m) return SR_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    for (Level &V : m->lv) {
+        (void)hipFree(V.u); (void)hipFree(V.f); (void)hipFree(V.t);
+        (void)hipFree(V.u32); (void)hipFree(V.f32); (void)hipFree(V.t32);
+        for (int a = 0; a < 3; a++) { (void)hipFree(V.c64[a]); (void)hipFree(V.c32[a]); (void)hipFree(V.wc[a]); }
+    }
+    for (int q = 0; q < 6; q++) (void)hipFree(m->wall[q]);
+    for (int a = 0; a < 3; a++) (void)hipFree(m->wvec[a]);
+    (void)hipFree(m->part); (void)hipFree(m->dres); (void)hipFree(m->diters); (void)hipFree(m->stage);
+    (void)hipFree(m->stage2); (void)hipFree(m->r32);
+    if (m->hres) (void)hipHostFree(m->hres);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->sw0) (void)hipEventDestroy(m->sw0);
+    if (m->sw1) (void)hipEventDestroy(m->sw1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// boundary rules
+// ------------------------------------------------------------------------------------------------
+// imposeBC rules of pressureData at a level (poisson_d3.cc:575-668)
+Bc sr::pressure_bc(const sr_mg *m, int level, int zeroBC) {
+    Bc b{};
+    int per[3] = {m->d.xPer, m->d.yPer, m->d.zPer};
+    for (int a = 0; a < 3; a++) {
+        if (per[a]) b.rule[a] = G_PERIODIC;
+        else if (m->d.testPoisson) b.rule[a] = zeroBC ? G_ANTI : G_WALL;
+        else b.rule[a] = G_MIRROR;
+    }
+    for (int q = 0; q < 6; q++) b.wall[q] = (level == 0) ? m->wall[q] : nullptr;
+    return b;
+}
+
+// updatePads rules (poisson_d3.cc:671-686): periodic x/y exchange, nothing else touched
+static Bc pads_bc(const sr_mg *m) {
+    Bc b{};
+    b.rule[0] = m->d.xPer ? G_PERIODIC : G_KEEP;
+    b.rule[1] = m->d.yPer ? G_PERIODIC : G_KEEP;
+    b.rule[2] = G_KEEP;
+    return b;
+}
+
+// residual pads read by coarsen(): reference = zero except the periodic x/y exchange
+// (poisson_d3.cc:91, 110); padMode 1 = follow each direction's ghost rule
+Bc sr::tmp_bc(const sr_mg *m) {
+    Bc b{};
+    int per[3] = {m->d.xPer, m->d.yPer, m->d.zPer};
+    for (int a = 0; a < 3; a++) {
+        if (m->d.padMode == 1) {
+            if (per[a]) b.rule[a] = G_PERIODIC;
+            else b.rule[a] = m->d.testPoisson ? G_ZERO : G_MIRROR;
+        } else {
+            b.rule[a] = (a < 2 && per[a]) ? G_PERIODIC : G_ZERO;
+        }
+    }
+    return b;
+}
+
+static bool singular(const sr_mg *m) {
+    return !(m->d.testPoisson && !(m->d.xPer && m->d.yPer && m->d.zPer));
+}
+
+// ------------------------------------------------------------------------------------------------
+// operators (device, FP64 hierarchy)
+// ------------------------------------------------------------------------------------------------
+// smooth(): ghosts, then `count` red-black sweeps (each updated node refreshes its face ghosts);
+// exact = 1 adds the closing imposeBC of poisson_d3.cc:181 (edge and corner ghosts included).
+void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact, hipEvent_t ea,
+                      hipEvent_t eb) {
+    Level &V = m->lv[l];
+    Bc bc = pressure_bc(m, l, zeroBC);
+    hipStream_t s = m->stream;
+    if (m->d.gsSmooth) {
+        if (!zero_start) launch_fill_ghosts<double>(V.L, V.u, bc, s);
+        if (ea) (void)hipEventRecord(ea, s);
+        for (int n = 0; n < count; n++) {
+            launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 0, bc, zero_start && n == 0, s);
+            launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 1, bc, false, s);
+        }
+        if (eb) (void)hipEventRecord(eb, s);
+    } else {
+        // Jacobi: tmp(l) = 0; per sweep {imposeBC; tmp = J(u); swap hierarchies} (poisson_d3.cc:152-168)
+        launch_fill<double>(V.t, lay_size(V.L), 0.0, s);
+        for (int n = 0; n < count; n++) {
+            Level &W = m->lv[l];
+            launch_fill_ghosts<double>(W.L, W.u, bc, s);
+            launch_jacobi<double>(W.L, W.u, W.f, W.t, W.c64[0], W.c64[1], W.c64[2], s);
+            for (Level &X : m->lv) std::swap(X.u, X.t);
+        }
+    }
+    if (exact || !m->d.gsSmooth) launch_fill_ghosts<double>(V.L, m->lv[l].u, bc, s);
+}
+
+void sr::residual_level(sr_mg *m, int l) {
+    Level &V = m->lv[l];
+    launch_residual<double, double>(V.L, V.u, V.f, V.t, V.c64[0], V.c64[1], V.c64[2], m->stream);
+    launch_fill_ghosts<double>(V.L, V.t, tmp_bc(m), m->stream);
+}
+
+void sr::coarsest_level(sr_mg *m, int l, int zeroBC) {
+    Level &V = m->lv[l];
+    Bc bc = pressure_bc(m, l, zeroBC);
+    launch_coarsest<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], bc, m->d.mgTolerance, m->maxCount,
+                            m->diters, m->stream);
+}
+
+// range projection of the coarsest right-hand side with that level's own null vectors
+static void project_coarsest(sr_mg *m) {
+    Level &Cv = m->lv[m->depth];
+    launch_project_small<double>(Cv.L, Cv.f, Cv.wc[0], Cv.wc[1], Cv.wc[2], Cv.wcsum, m->stream);
+}
+
+// poisson.cc:219-280 with restriction fused into the down leg and += smoothedPres into prolong
+void sr::vcycle_fp64(sr_mg *m) {
+    const sr_mg_desc &d = m->d;
+    int D = m->depth;
+    smooth_level(m, 0, d.preSmooth, 0, false, false, m->sw0, m->sw1);
+    for (int l = 0; l < D; l++) {
+        residual_level(m, l);
+        launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream);
+        if (l + 1 == D) {
+            launch_fill<double>(m->lv[D].u, lay_size(m->lv[D].L), 0.0, m->stream);
+            if (d.coarseProject && singular(m)) project_coarsest(m);
+            coarsest_level(m, D, 1);
+        } else if (d.gsSmooth) {
+            smooth_level(m, l + 1, d.preSmooth, 1, true, false);
+        } else {
+            launch_fill<double>(m->lv[l + 1].u, lay_size(m->lv[l + 1].L), 0.0, m->stream);
+            smooth_level(m, l + 1, d.preSmooth, 1, false, false);
+        }
+    }
+    for (int l = D - 1; l >= 0; l--) {
+        launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream);
+        smooth_level(m, l, d.postSmooth, l == 0 ? 0 : 1, false, false);
+    }
+}
+
+void sr::norm_level0(sr_mg *m) {
+    Level &V = m->lv[0];
+    launch_norm<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part, m->dres, m->stream);
+}
+
+// computeError from the four reduced quantities (poisson_d3.cc:433-458)
+double sr::error_from(const sr_mg *m, const double *q, int norm) {
+    const Lay &L = m->lv[0].L;
+    double cnt = (double)L.nx * L.ny * L.nz;
+    if (norm == 1) return q[3] != 0.0 ? q[2] / q[3] : q[2];
+    if (norm == 2) return q[1] != 0.0 ? std::sqrt(q[0] / cnt) / std::sqrt(q[1] / cnt) : std::sqrt(q[0] / cnt);
+    return 0.0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// per-operator entry points
+// ------------------------------------------------------------------------------------------------
+static double *level_array(sr_mg *m, int which, int level) {
+    Level &V = m->lv[level];
+    return which == SR_PRESSURE ? V.u : which == SR_RESIDUAL ? V.f : V.t;
+}
+
+extern "C" int sr_mg_set_level(sr_mg *m, int level) {
+    if (!m || level < 0 || level > m->depth) return set_err(SR_ERR_ARG, "level out of range");
+    m->vLevel = level;
+    return SR_OK;
+}
+extern "C" int sr_mg_get_level(const sr_mg *m, int *level) {
+    if (!m || !level) return set_err(SR_ERR_ARG, "null argument");
+    *level = m->vLevel;
+    return SR_OK;
+}
+extern "C" int sr_mg_set_zero_bc(sr_mg *m, int z) {
+    if (!m) return set_err(SR_ERR_ARG, "null argument");
+    m->zeroBC = z ? 1 : 0;
+    return SR_OK;
+}
+extern "C" int sr_mg_level_dims(const sr_mg *m, int level, int *nx, int *ny, int *nz) {
+    if (!m || level < 0 || level > m->depth) return set_err(SR_ERR_ARG, "level out of range");
+    *nx = m->lv[level].L.nx;
+    *ny = m->lv[level].L.ny;
+    *nz = m->lv[level].L.nz;
+    return SR_OK;
+}
+
+static int ensure_stage(sr_mg *m) {
+    if (m->stage) return SR_OK;
+    const Lay &L = m->lv[0].L;
+    size_t n = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    HIPCHK(hipMalloc((void **)&m->stage, sizeof(double) * n));
+    HIPCHK(hipMalloc((void **)&m->stage2, sizeof(double) * n));
+    return SR_OK;
+}
+
+extern "C" int sr_mg_upload(sr_mg *m, int which, int level, const double *host) {
+    if (!m || !host || level < 0 || level > m->depth || which < 0 || which > 2)
+        return set_err(SR_ERR_ARG, "bad upload arguments");
+    RETCHK(ensure_stage(m));
+    const Lay &L = m->lv[level].L;
+    size_t n = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    HIPCHK(hipMemcpyAsync(m->stage, host, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+    launch_pack<double, double>(L, m->stage, level_array(m, which, level), 0, m->stream);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return SR_OK;
+}
+
+extern "C" int sr_mg_download(sr_mg *m, int which, int level, double *host) {
+    if (!m || !host || level < 0 || level > m->depth || which < 0 || which > 2)
+        return set_err(SR_ERR_ARG, "bad download arguments");
+    RETCHK(ensure_stage(m));
+    const Lay &L = m->lv[level].L;
+    size_t n = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    launch_unpack<double, double>(L, level_array(m, which, level), m->stage, m->stream);
+    HIPCHK(hipMemcpyAsync(host, m->stage, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return SR_OK;
+}
+
+static int sync_check(sr_mg *m) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return SR_OK;
+}
+
+static int wall_guard(const sr_mg *m) {
+    if (m->d.testPoisson && !m->zeroBC && m->vLevel != 0)
+        return set_err(SR_ERR_STATE, "non-zero Dirichlet walls exist on level 0 only");
+    return SR_OK;
+}
+
+extern "C" int sr_mg_op_smooth(sr_mg *m, int count) {
+    if (!m || count < 0) return set_err(SR_ERR_ARG, "bad smooth arguments");
+    RETCHK(wall_guard(m));
+    smooth_level(m, m->vLevel, count, m->zeroBC, false, true);
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_compute_residual(sr_mg *m) {
+    if (!m) return set_err(SR_ERR_ARG, "null");
+    residual_level(m, m->vLevel);
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_coarsen(sr_mg *m) {
+    if (!m || m->vLevel >= m->depth) return set_err(SR_ERR_STATE, "coarsen below the coarsest level");
+    int l = m->vLevel;
+    launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream);
+    m->vLevel += 1;
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_prolong(sr_mg *m) {
+    if (!m || m->vLevel < 1) return set_err(SR_ERR_STATE, "prolong above the finest level");
+    int l = m->vLevel - 1;
+    // pressureData(vLevel) = 0.0 over the padded level, then the core (poisson_d3.cc:307-309)
+    launch_fill<double>(m->lv[l].u, lay_size(m->lv[l].L), 0.0, m->stream);
+    launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 0, m->stream);
+    m->vLevel = l;
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_solve_coarsest(sr_mg *m) {
+    if (!m) return set_err(SR_ERR_ARG, "null");
+    if (coarsest_lds_bytes<double>(m->lv[m->vLevel].L) > 60 * 1024)
+        return set_err(SR_ERR_ARG, "level too large for the single-workgroup solve");
+    RETCHK(wall_guard(m));
+    coarsest_level(m, m->vLevel, m->zeroBC);
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_impose_bc(sr_mg *m) {
+    if (!m) return set_err(SR_ERR_ARG, "null");
+    RETCHK(wall_guard(m));
+    Level &V = m->lv[m->vLevel];
+    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, m->vLevel, m->zeroBC), m->stream);
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_update_pads(sr_mg *m, int which) {
+    if (!m || which < 0 || which > 2) return set_err(SR_ERR_ARG, "bad array");
+    Level &V = m->lv[m->vLevel];
+    launch_fill_ghosts<double>(V.L, level_array(m, which, m->vLevel), pads_bc(m), m->stream);
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_op_compute_error(sr_mg *m, int norm, double *out) {
+    if (!m || !out) return set_err(SR_ERR_ARG, "null");
+    if (m->vLevel != 0) return set_err(SR_ERR_STATE, "computeError runs on the finest level");
+    norm_level0(m);
+    HIPCHK(hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream));
+    RETCHK(sync_check(m));
+    *out = error_from(m, m->hres, norm);
+    return SR_OK;
+}
+
+extern "C" int sr_mg_op_vcycle(sr_mg *m) {
+    if (!m) return set_err(SR_ERR_ARG, "null");
+    vcycle_fp64(m);
+    m->vLevel = 0;
+    m->zeroBC = 0;
+    return sync_check(m);
+}
+
+extern "C" int sr_mg_history(const sr_mg *m, double *out, int cap) {
+    if (!m) return -1;
+    int n = std::min<int>(cap, (int)m->history.size());
+    for (int i = 0; i < n; i++) out[i] = m->history[i];
+    return n;
+}

@@ -1,0 +1,40 @@
+// Host-side launchers of the multigrid kernels (mg_kernels.hip).  All launches go to the given
+// stream; none allocates memory or synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mg_layout.h"
+
+namespace sr {
+
+template <typename T> void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s);
+template <typename T>
+void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
+                     const Bc &bc, bool zero_start, hipStream_t s);
+template <typename T, typename TO>
+void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
+                     hipStream_t s);
+template <typename T> void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s);
+template <typename T, typename TF>
+void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s);
+template <typename T> void launch_fill(T *a, long n, T v, hipStream_t s);
+template <typename T>
+void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int core, hipStream_t s);
+template <typename TS, typename TD> void launch_pack(const Lay &L, const TS *ref, TD *dst, int core_only, hipStream_t s);
+template <typename TS, typename TD> void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s);
+int norm_parts(const Lay &L);
+// out[0..3] = sum |Lu-f|^2, sum f^2 (core), max |Lu-f| (core), max |f| (every padded node)
+template <typename T>
+void launch_norm(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part,
+                 double *out, hipStream_t s);
+// out[0] = sum_core w_i w_j w_k f, out[1] = sum over every padded node of f
+template <typename T>
+void launch_rhs_sums(const Lay &L, const T *f, const double *wx, const double *wy, const double *wz, double *part,
+                     double *out, hipStream_t s);
+template <typename T> size_t coarsest_lds_bytes(const Lay &L);
+template <typename T>
+void launch_coarsest(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, const Bc &bc,
+                     double tol, int max_count, int *iters, hipStream_t s);
+
+}  // namespace sr

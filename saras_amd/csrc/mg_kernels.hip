@@ -1,0 +1,749 @@
+// gfx950 kernels of the multigrid V-cycle on the red-black split layout (mg_layout.h).
+//
+// Each kernel restates one operator of lib/poisson/poisson_d3.cc on one level:
+//   k_rb_sweep       smooth() Gauss-Seidel sweep (poisson_d3.cc:137-150), red-black order, with the
+//                    face-ghost images of every updated node written by the updating thread
+//   k_fill_ghosts    imposeBC() / updatePads() (poisson_d3.cc:575-686), composed per ghost node
+//   k_residual       computeResidual() (poisson_d3.cc:90-111)
+//   k_restrict       coarsen(): 27-point full weighting (poisson_d3.cc:250-298)
+//   k_prolong        prolong() (poisson_d3.cc:301-388), optionally fused with += smoothedPres
+//   k_norm_*         computeError() (poisson_d3.cc:391-459): deterministic two-pass reductions
+//   k_coarsest       solve() (poisson_d3.cc:190-247): the whole coarsest-level iteration in LDS
+//
+// The discrete operator is the reference's, rewritten with per-index coefficients
+//   L u = sum_d lp_d u(+1) + lm_d u(-1) - lc_d u(0),
+//   lp = xix2/h^2 + xixx/(2h),  lm = xix2/h^2 - xixx/(2h),  lc = 2 xix2/h^2,
+// and the Gauss-Seidel update u = (sum_d lp_d u(+1) + lm_d u(-1) - f) / sum_d lc_d, which equals the
+// reference quotient of poisson_d3.cc:140-147 after dividing numerator and denominator by 2 hx^2hy^2hz^2.
+#include "mg_kernels.h"
+#include "mg_state.h"
+
+#include <hip/hip_runtime.h>
+
+namespace sr {
+
+// Workgroups are dealt round-robin to the 8 XCDs; remap so that consecutive logical blocks (which
+// touch neighbouring columns) run on the same XCD and share its L2.
+__device__ inline unsigned xcd_block() {
+    unsigned nb = gridDim.x, b = blockIdx.x;
+    unsigned per = nb >> 3;
+    if (b < per * 8u) return (b & 7u) * per + (b >> 3);
+    return b;
+}
+
+// ------------------------------------------------------------------------------------------------
+// ghost composition (imposeBC order x, y, z; z outermost)
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline bool ghost_resolve(const Lay &L, const T *a, int ip, int jp, int kp, const Bc &bc, T &out) {
+    int n[3] = {L.nx, L.ny, L.nz};
+    int q[3] = {ip, jp, kp};
+    T sign = T(1);
+    bool moved = false;
+    for (int d = 2; d >= 0; --d) {
+        int g = q[d];
+        if (g != 0 && g != n[d] + 1) continue;
+        int r = bc.rule[d];
+        if (r == G_KEEP) continue;
+        if (r == G_ZERO) { out = T(0); return true; }
+        if (r == G_WALL) {
+            const double *w = bc.wall[2 * d + (g == 0 ? 0 : 1)];
+            T v = T(0);
+            if (w) {
+                long wi = d == 0 ? (long)q[1] * (L.nz + 2) + q[2]
+                        : d == 1 ? (long)q[0] * (L.nz + 2) + q[2]
+                                 : (long)q[0] * (L.ny + 2) + q[1];
+                v = (T)w[wi];
+            }
+            out = sign * v;
+            return true;
+        }
+        q[d] = ghost_src(r, g, n[d]);
+        if (r == G_ANTI) sign = -sign;
+        moved = true;
+    }
+    if (!moved) return false;
+    out = sign * a[lay_idx(L, q[0], q[1], q[2])];
+    return true;
+}
+
+// One thread per ghost node: z faces (full ip, jp), then y faces (full ip, core kp), then x faces.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_ghosts(Lay L, T *__restrict__ a, Bc bc, long total) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int NX = L.nx + 2, NY = L.ny + 2;
+    long nzf = 2L * NX * NY;
+    long nyf = 2L * NX * L.nz;
+    int ip, jp, kp;
+    if (t < nzf) {
+        long r = t;
+        kp = (r & 1) ? L.nz + 1 : 0;
+        r >>= 1;
+        jp = (int)(r % NY);
+        ip = (int)(r / NY);
+    } else if (t < nzf + nyf) {
+        long r = t - nzf;
+        kp = 1 + (int)(r % L.nz);
+        r /= L.nz;
+        jp = (r & 1) ? L.ny + 1 : 0;
+        ip = (int)(r >> 1);
+    } else {
+        long r = t - nzf - nyf;
+        kp = 1 + (int)(r % L.nz);
+        r /= L.nz;
+        jp = 1 + (int)(r % L.ny);
+        r /= L.ny;
+        ip = r ? L.nx + 1 : 0;
+    }
+    T v;
+    if (ghost_resolve(L, a, ip, jp, kp, bc, v)) a[lay_idx(L, ip, jp, kp)] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// red-black Gauss-Seidel half sweep
+// ------------------------------------------------------------------------------------------------
+// Face-ghost images of an updated node: a node whose padded index is the source of a ghost of a
+// periodic / mirror / anti direction writes that ghost (same colour, so no other thread of this
+// half sweep reads it).  Edge and corner ghosts are not read by the 7-point stencil.
+template <typename T>
+__device__ inline void write_images(const Lay &L, T *u, int ip, int jp, int kp, T v, const Bc &bc) {
+    int n[3] = {L.nx, L.ny, L.nz};
+    int p[3] = {ip, jp, kp};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int r = bc.rule[d];
+        if (r > G_ANTI) continue;
+        T w = (r == G_ANTI) ? -v : v;
+        int g0 = ghost_src(r, 0, n[d]), g1 = ghost_src(r, n[d] + 1, n[d]);
+        if (p[d] == g0) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[d] = 0;
+            u[lay_idx(L, q[0], q[1], q[2])] = w;
+        }
+        if (p[d] == g1) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[d] = n[d] + 1;
+            u[lay_idx(L, q[0], q[1], q[2])] = w;
+        }
+    }
+}
+
+// ZERO_START: every opposite-colour value is known to be zero (first red half sweep after the level
+// was cleared), so the update reduces to -f / sum lc and no u is read.
+template <typename T, bool ZERO_START>
+__global__ void __launch_bounds__(256) k_rb_sweep(Lay L, T *__restrict__ u, const T *__restrict__ f,
+                                                  const T *__restrict__ cx, const T *__restrict__ cy,
+                                                  const T *__restrict__ cz, int color, Bc bc) {
+    unsigned t = xcd_block() * 256u + threadIdx.x;
+    unsigned per_plane = (unsigned)L.ny * (unsigned)L.kh;
+    unsigned ipl = t / per_plane;
+    unsigned rem = t - ipl * per_plane;
+    int ip = 1 + (int)ipl;
+    if (ip > L.nx) return;
+    int jp = 1 + (int)(rem / (unsigned)L.kh);
+    int h = (int)(rem % (unsigned)L.kh);
+    int kb = (color + 1 + ip + jp) & 1;
+    int kp = 2 * h + kb;
+    if (kp < 1 || kp > L.nz) return;
+    int o = color ^ 1;
+    long rc = lay_row(L, ip, jp, color);
+    T num;
+    const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+    T diag = X[2] + Y[2] + Z[2];
+    if (ZERO_START) {
+        num = -f[rc + h];
+    } else {
+        long ro = lay_row(L, ip, jp, o);
+        long sx = 2L * (L.ny + 2) * L.khs;
+        long sy = 2L * L.khs;
+        T xm = u[ro - sx + h], xp = u[ro + sx + h];
+        T ym = u[ro - sy + h], yp = u[ro + sy + h];
+        T zm = u[ro + h - 1 + kb], zp = u[ro + h + kb];
+        num = X[0] * xp + X[1] * xm + Y[0] * yp + Y[1] * ym + Z[0] * zp + Z[1] * zm - f[rc + h];
+    }
+    T v = num / diag;
+    u[rc + h] = v;
+    write_images(L, u, ip, jp, kp, v, bc);
+}
+
+// ------------------------------------------------------------------------------------------------
+// residual t = f - L u on core nodes (both colours)
+// ------------------------------------------------------------------------------------------------
+template <typename T, typename TO>
+__device__ inline T apply_lap(const Lay &L, const T *__restrict__ u, long row_c, long row_o, int h, int kb,
+                              const T *X, const T *Y, const T *Z) {
+    long sx = 2L * (L.ny + 2) * L.khs;
+    long sy = 2L * L.khs;
+    T c = u[row_c + h];
+    T xm = u[row_o - sx + h], xp = u[row_o + sx + h];
+    T ym = u[row_o - sy + h], yp = u[row_o + sy + h];
+    T zm = u[row_o + h - 1 + kb], zp = u[row_o + h + kb];
+    return X[0] * xp + X[1] * xm + Y[0] * yp + Y[1] * ym + Z[0] * zp + Z[1] * zm - (X[2] + Y[2] + Z[2]) * c;
+}
+
+template <typename T, typename TO>
+__global__ void __launch_bounds__(256) k_residual(Lay L, const T *__restrict__ u, const T *__restrict__ f,
+                                                  TO *__restrict__ t_out, const T *__restrict__ cx,
+                                                  const T *__restrict__ cy, const T *__restrict__ cz) {
+    unsigned t = xcd_block() * 256u + threadIdx.x;
+    unsigned per_plane = (unsigned)L.ny * 2u * (unsigned)L.kh;
+    unsigned ipl = t / per_plane;
+    unsigned rem = t - ipl * per_plane;
+    int ip = 1 + (int)ipl;
+    if (ip > L.nx) return;
+    unsigned jr = rem / (2u * (unsigned)L.kh);
+    unsigned r2 = rem - jr * 2u * (unsigned)L.kh;
+    int jp = 1 + (int)jr;
+    int c = (int)(r2 / (unsigned)L.kh);
+    int h = (int)(r2 % (unsigned)L.kh);
+    int kb = (c + 1 + ip + jp) & 1;
+    int kp = 2 * h + kb;
+    if (kp < 1 || kp > L.nz) return;
+    long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+    T lu = apply_lap<T, TO>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+    t_out[rc + h] = (TO)(f[rc + h] - lu);
+}
+
+// ------------------------------------------------------------------------------------------------
+// restriction: coarse f(I,J,K) = full weighting of fine t around (2I, 2J, 2K), ghosts included
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, const T *__restrict__ t, T *__restrict__ fc) {
+    unsigned g = xcd_block() * 256u + threadIdx.x;
+    unsigned per_plane = (unsigned)Cl.ny * 2u * (unsigned)Cl.kh;
+    unsigned ipl = g / per_plane;
+    unsigned rem = g - ipl * per_plane;
+    int ip = 1 + (int)ipl;
+    if (ip > Cl.nx) return;
+    unsigned jr = rem / (2u * (unsigned)Cl.kh);
+    unsigned r2 = rem - jr * 2u * (unsigned)Cl.kh;
+    int jp = 1 + (int)jr;
+    int c = (int)(r2 / (unsigned)Cl.kh);
+    int h = (int)(r2 % (unsigned)Cl.kh);
+    int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+    if (kp < 1 || kp > Cl.nz) return;
+    // fine padded index of the coincident node: logical 2i -> padded 2(ip-1)+1
+    int fi = 2 * ip - 1, fj = 2 * jp - 1, fk = 2 * kp - 1;
+#define TT(a, b, e) t[lay_idx(F, fi + (a), fj + (b), fk + (e))]
+    T facePoints = (TT(1, 0, 0) + TT(-1, 0, 0) + TT(0, 1, 0) + TT(0, -1, 0) + TT(0, 0, 1) + TT(0, 0, -1)) * T(0.0625);
+    T edgePoints = (TT(1, 1, 0) + TT(1, -1, 0) + TT(-1, -1, 0) + TT(-1, 1, 0) + TT(0, 1, 1) + TT(0, -1, 1) +
+                    TT(0, -1, -1) + TT(0, 1, -1) + TT(1, 0, 1) + TT(1, 0, -1) + TT(-1, 0, -1) + TT(-1, 0, 1)) *
+                   T(0.03125);
+    T vertPoints = (TT(1, 1, 1) + TT(1, 1, -1) + TT(1, -1, 1) + TT(-1, 1, 1) + TT(1, -1, -1) + TT(-1, 1, -1) +
+                    TT(-1, -1, 1) + TT(-1, -1, -1)) *
+                   T(0.015625);
+    fc[lay_row(Cl, ip, jp, c) + h] = facePoints + edgePoints + vertPoints + TT(0, 0, 0) * T(0.125);
+#undef TT
+}
+
+// ------------------------------------------------------------------------------------------------
+// prolongation: fine u (+)= trilinear interpolation of coarse e (core nodes only)
+// ------------------------------------------------------------------------------------------------
+template <typename T, typename TF>
+__global__ void __launch_bounds__(256) k_prolong(Lay F, Lay Cl, const T *__restrict__ e, TF *__restrict__ u,
+                                                 int accumulate) {
+    unsigned g = xcd_block() * 256u + threadIdx.x;
+    unsigned per_plane = (unsigned)F.ny * 2u * (unsigned)F.kh;
+    unsigned ipl = g / per_plane;
+    unsigned rem = g - ipl * per_plane;
+    int ip = 1 + (int)ipl;
+    if (ip > F.nx) return;
+    unsigned jr = rem / (2u * (unsigned)F.kh);
+    unsigned r2 = rem - jr * 2u * (unsigned)F.kh;
+    int jp = 1 + (int)jr;
+    int c = (int)(r2 / (unsigned)F.kh);
+    int h = (int)(r2 % (unsigned)F.kh);
+    int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+    if (kp < 1 || kp > F.nz) return;
+    int i = ip - 1, j = jp - 1, k = kp - 1;
+    int io = i & 1, jo = j & 1, ko = k & 1;
+    int ci = i / 2 + 1, cj = j / 2 + 1, ck = k / 2 + 1;   // coarse padded index of i2, j2, k2
+#define EE(a, b, q) e[lay_idx(Cl, ci + (a), cj + (b), ck + (q))]
+    T v;
+    if (io && jo && ko)
+        v = (EE(0, 0, 0) + EE(1, 0, 0) + EE(0, 1, 0) + EE(0, 0, 1) + EE(1, 1, 0) + EE(1, 0, 1) + EE(0, 1, 1) +
+             EE(1, 1, 1)) / T(8.0);
+    else if (io && jo)
+        v = (EE(0, 0, 0) + EE(0, 1, 0) + EE(1, 0, 0) + EE(1, 1, 0)) / T(4.0);
+    else if (io && ko)
+        v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(1, 0, 0) + EE(1, 0, 1)) / T(4.0);
+    else if (io)
+        v = (EE(0, 0, 0) + EE(1, 0, 0)) / T(2.0);
+    else if (jo && ko)
+        v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(0, 1, 0) + EE(0, 1, 1)) / T(4.0);
+    else if (jo)
+        v = (EE(0, 0, 0) + EE(0, 1, 0)) / T(2.0);
+    else if (ko)
+        v = (EE(0, 0, 0) + EE(0, 0, 1)) / T(2.0);
+    else
+        v = EE(0, 0, 0);
+#undef EE
+    long o = lay_row(F, ip, jp, c) + h;
+    if (accumulate) u[o] = (TF)((TF)u[o] + (TF)v);
+    else u[o] = (TF)v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// fills and scalar updates
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T *__restrict__ a, long n, T v) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) a[t] = v;
+}
+
+// a(node) += v over core nodes (core = 1) or over every padded node (core = 0)
+template <typename T>
+__global__ void __launch_bounds__(256) k_add_scalar(Lay L, T *__restrict__ a, const double *__restrict__ dv,
+                                                    double scale, int core) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = lay_size(L);
+    if (t >= n) return;
+    int h = (int)(t % L.khs);
+    long r = t / L.khs;
+    int c = (int)(r & 1);
+    r >>= 1;
+    int jp = (int)(r % (L.ny + 2));
+    int ip = (int)(r / (L.ny + 2));
+    if (h >= L.kh) return;
+    int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+    if (kp > L.nz + 1) return;
+    if (core && (ip < 1 || ip > L.nx || jp < 1 || jp > L.ny || kp < 1 || kp > L.nz)) return;
+    a[t] = (T)((double)a[t] + scale * dv[0]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// layout conversion between the reference padded array and the split layout
+// ------------------------------------------------------------------------------------------------
+// core_only = 1: copy core nodes only (ghost slots untouched)
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) k_pack(Lay L, const TS *__restrict__ ref, TD *__restrict__ dst, int core_only) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long NZ = L.nz + 2, NY = L.ny + 2;
+    long n = (long)(L.nx + 2) * NY * NZ;
+    if (t >= n) return;
+    int kp = (int)(t % NZ);
+    long r = t / NZ;
+    int jp = (int)(r % NY);
+    int ip = (int)(r / NY);
+    if (core_only && (ip < 1 || ip > L.nx || jp < 1 || jp > L.ny || kp < 1 || kp > L.nz)) return;
+    dst[lay_idx(L, ip, jp, kp)] = (TD)ref[t];
+}
+
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) k_unpack(Lay L, const TS *__restrict__ src, TD *__restrict__ ref) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long NZ = L.nz + 2, NY = L.ny + 2;
+    long n = (long)(L.nx + 2) * NY * NZ;
+    if (t >= n) return;
+    int kp = (int)(t % NZ);
+    long r = t / NZ;
+    int jp = (int)(r % NY);
+    int ip = (int)(r / NY);
+    ref[t] = (TD)src[lay_idx(L, ip, jp, kp)];
+}
+
+// ------------------------------------------------------------------------------------------------
+// reductions (deterministic: fixed per-block order, then one block over the partials)
+// ------------------------------------------------------------------------------------------------
+constexpr int NQ = 4;   // quantities per partial
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// q[0] += sum, q[1] += sum, q[2] = max, q[3] = max over the block; written to part[block*NQ + i]
+__device__ inline void block_reduce4(double s0, double s1, double m2, double m3, double *part) {
+    __shared__ double sh[4][4];
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    m2 = wave_max(m2);
+    m3 = wave_max(m3);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[w][0] = s0; sh[w][1] = s1; sh[w][2] = m2; sh[w][3] = m3; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        double a = 0, b = 0, c = 0, d = 0;
+        for (int i = 0; i < nw; ++i) {
+            a += sh[i][0];
+            b += sh[i][1];
+            c = fmax(c, sh[i][2]);
+            d = fmax(d, sh[i][3]);
+        }
+        double *p = part + (long)blockIdx.x * NQ;
+        p[0] = a; p[1] = b; p[2] = c; p[3] = d;
+    }
+}
+
+// level-0 residual norms: q0 = sum |Lu - f|^2, q1 = sum f^2 (core), q2 = max |Lu - f|,
+// q3 = max |f| over every padded node (computeError(1) denominator, poisson_d3.cc:437)
+template <typename T>
+__global__ void __launch_bounds__(256) k_norm_partial(Lay L, const T *__restrict__ u, const T *__restrict__ f,
+                                                      const T *__restrict__ cx, const T *__restrict__ cy,
+                                                      const T *__restrict__ cz, double *__restrict__ part) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = lay_size(L);
+    double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
+    if (t < n) {
+        int h = (int)(t % L.khs);
+        long r = t / L.khs;
+        int c = (int)(r & 1);
+        r >>= 1;
+        int jp = (int)(r % (L.ny + 2));
+        int ip = (int)(r / (L.ny + 2));
+        int kb = (c + 1 + ip + jp) & 1;
+        int kp = 2 * h + kb;
+        if (h < L.kh && kp <= L.nz + 1) {
+            double fv = (double)f[t];
+            m3 = fabs(fv);
+            if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
+                long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+                double lu = (double)apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+                double v = fabs(lu - fv);
+                s0 = v * v;
+                s1 = fv * fv;
+                m2 = v;
+            }
+        }
+    }
+    block_reduce4(s0, s1, m2, m3, part);
+}
+
+// weighted sums over the reference right-hand side: q0 = sum w_i w_j w_k f (core), q1 = sum f over
+// every padded node
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs_sums(Lay L, const T *__restrict__ f, const double *__restrict__ wx,
+                                                  const double *__restrict__ wy, const double *__restrict__ wz,
+                                                  double *__restrict__ part) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = lay_size(L);
+    double s0 = 0, s1 = 0;
+    if (t < n) {
+        int h = (int)(t % L.khs);
+        long r = t / L.khs;
+        int c = (int)(r & 1);
+        r >>= 1;
+        int jp = (int)(r % (L.ny + 2));
+        int ip = (int)(r / (L.ny + 2));
+        int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+        if (h < L.kh && kp <= L.nz + 1) {
+            double fv = (double)f[t];
+            s1 = fv;
+            if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz && wx)
+                s0 = wx[ip - 1] * wy[jp - 1] * wz[kp - 1] * fv;
+        }
+    }
+    block_reduce4(s0, s1, 0.0, 0.0, part);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_final(const double *__restrict__ part, int nparts, double *__restrict__ out) {
+    double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const double *p = part + (long)i * NQ;
+        s0 += p[0];
+        s1 += p[1];
+        m2 = fmax(m2, p[2]);
+        m3 = fmax(m3, p[3]);
+    }
+    // single block: its one "partial" is the result
+    block_reduce4(s0, s1, m2, m3, out);
+}
+
+// ------------------------------------------------------------------------------------------------
+// coarsest level: the whole solve() loop in LDS, one workgroup
+// ------------------------------------------------------------------------------------------------
+// Every composed ghost value reads core nodes only, so the writes need no ordering among themselves.
+template <typename T>
+__device__ inline void lds_fill_ghosts(const Lay &L, T *a, const Bc &bc) {
+    int NX = L.nx + 2, NY = L.ny + 2, NZ = L.nz + 2;
+    int n = NX * NY * NZ;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int kp = t % NZ, r = t / NZ, jp = r % NY, ip = r / NY;
+        bool ghost = ip == 0 || ip == NX - 1 || jp == 0 || jp == NY - 1 || kp == 0 || kp == NZ - 1;
+        T v;
+        if (ghost && ghost_resolve(L, a, ip, jp, kp, bc, v)) a[lay_idx(L, ip, jp, kp)] = v;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline void lds_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color) {
+    int n = L.nx * L.ny * L.nz;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        if (((i + j + k) & 1) != color) continue;
+        int ip = i + 1, jp = j + 1, kp = k + 1;
+        const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        T num = X[0] * u[lay_idx(L, ip + 1, jp, kp)] + X[1] * u[lay_idx(L, ip - 1, jp, kp)] +
+                Y[0] * u[lay_idx(L, ip, jp + 1, kp)] + Y[1] * u[lay_idx(L, ip, jp - 1, kp)] +
+                Z[0] * u[lay_idx(L, ip, jp, kp + 1)] + Z[1] * u[lay_idx(L, ip, jp, kp - 1)] - f[lay_idx(L, ip, jp, kp)];
+        u[lay_idx(L, ip, jp, kp)] = num / (X[2] + Y[2] + Z[2]);
+    }
+}
+
+// Loop of poisson_d3.cc:190-247 with the red-black sweep: {ghosts; red; ghosts; black; max|r|}
+// until max|r| < tol or more than max_count iterations; ghosts once more at the end.
+// The per-iteration count is written to iters[0].
+template <typename T>
+__global__ void __launch_bounds__(256) k_coarsest(Lay L, T *__restrict__ ug, const T *__restrict__ fg,
+                                                  const T *__restrict__ cxg, const T *__restrict__ cyg,
+                                                  const T *__restrict__ czg, Bc bc, double tol, int max_count,
+                                                  int *__restrict__ iters) {
+    extern __shared__ unsigned char smem[];
+    long n = lay_size(L);
+    T *u = (T *)smem;
+    T *f = u + n;
+    T *cx = f + n;
+    T *cy = cx + 4 * (L.nx + 2);
+    T *cz = cy + 4 * (L.ny + 2);
+    for (long t = threadIdx.x; t < n; t += blockDim.x) { u[t] = ug[t]; f[t] = fg[t]; }
+    for (int t = threadIdx.x; t < 4 * (L.nx + 2); t += blockDim.x) cx[t] = cxg[t];
+    for (int t = threadIdx.x; t < 4 * (L.ny + 2); t += blockDim.x) cy[t] = cyg[t];
+    for (int t = threadIdx.x; t < 4 * (L.nz + 2); t += blockDim.x) cz[t] = czg[t];
+    __syncthreads();
+    __shared__ double red[4];
+    int it = 0;
+    while (true) {
+        lds_fill_ghosts(L, u, bc);
+        lds_sweep(L, u, f, cx, cy, cz, 0);
+        __syncthreads();
+        lds_fill_ghosts(L, u, bc);
+        lds_sweep(L, u, f, cx, cy, cz, 1);
+        __syncthreads();
+        double m = -1.0e-10;
+        int nc = L.nx * L.ny * L.nz;
+        for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+            int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+            int ip = i + 1, jp = j + 1, kp = k + 1;
+            int c = lay_color(ip, jp, kp);
+            long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+            double lu = (double)apply_lap<T, T>(L, u, rc, ro, kp >> 1, kp & 1, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+            m = fmax(m, fabs((double)f[rc + (kp >> 1)] - lu));
+        }
+        m = wave_max(m);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        double gm = red[0];
+        for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) gm = fmax(gm, red[w]);
+        __syncthreads();
+        if (gm < tol) break;
+        it += 1;
+        if (it > max_count) break;
+    }
+    lds_fill_ghosts(L, u, bc);
+    for (long t = threadIdx.x; t < n; t += blockDim.x) ug[t] = u[t];
+    if (threadIdx.x == 0) iters[0] = it;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Jacobi sweep t = J(u) on core nodes (poisson_d3.cc:152-166)
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_jacobi(Lay L, const T *__restrict__ u, const T *__restrict__ f,
+                                                T *__restrict__ tout, const T *__restrict__ cx,
+                                                const T *__restrict__ cy, const T *__restrict__ cz) {
+    unsigned g = xcd_block() * 256u + threadIdx.x;
+    unsigned per_plane = (unsigned)L.ny * 2u * (unsigned)L.kh;
+    unsigned ipl = g / per_plane;
+    unsigned rem = g - ipl * per_plane;
+    int ip = 1 + (int)ipl;
+    if (ip > L.nx) return;
+    unsigned jr = rem / (2u * (unsigned)L.kh);
+    unsigned r2 = rem - jr * 2u * (unsigned)L.kh;
+    int jp = 1 + (int)jr;
+    int c = (int)(r2 / (unsigned)L.kh);
+    int h = (int)(r2 % (unsigned)L.kh);
+    int kb = (c + 1 + ip + jp) & 1;
+    int kp = 2 * h + kb;
+    if (kp < 1 || kp > L.nz) return;
+    long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+    long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
+    const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+    T num = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
+            Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - f[rc + h];
+    tout[rc + h] = num / (X[2] + Y[2] + Z[2]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// range projection of a small level's right-hand side (one workgroup): f -= (w.f)/(w.1) on core
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_project_small(Lay L, T *__restrict__ f, const double *__restrict__ wx,
+                                                       const double *__restrict__ wy, const double *__restrict__ wz,
+                                                       double wsum) {
+    int n = L.nx * L.ny * L.nz;
+    double s = 0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        s += wx[i] * wy[j] * wz[k] * (double)f[lay_idx(L, i + 1, j + 1, k + 1)];
+    }
+    __shared__ double red[4];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double tot = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) tot += red[w];
+    double c = tot / wsum;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        long o = lay_idx(L, i + 1, j + 1, k + 1);
+        f[o] = (T)((double)f[o] - c);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// launch wrappers
+// ------------------------------------------------------------------------------------------------
+static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s) {
+    long total = 2L * (L.nx + 2) * (L.ny + 2) + 2L * (L.nx + 2) * L.nz + 2L * L.ny * L.nz;
+    hipLaunchKernelGGL(k_fill_ghosts<T>, dim3(nblk(total)), dim3(256), 0, s, L, a, bc, total);
+}
+
+template <typename T>
+void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
+                     const Bc &bc, bool zero_start, hipStream_t s) {
+    long total = (long)L.nx * L.ny * L.kh;
+    if (zero_start)
+        hipLaunchKernelGGL((k_rb_sweep<T, true>), dim3(nblk(total)), dim3(256), 0, s, L, u, f, cx, cy, cz, color, bc);
+    else
+        hipLaunchKernelGGL((k_rb_sweep<T, false>), dim3(nblk(total)), dim3(256), 0, s, L, u, f, cx, cy, cz, color, bc);
+}
+
+template <typename T, typename TO>
+void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
+                     hipStream_t s) {
+    long total = (long)L.nx * L.ny * 2 * L.kh;
+    hipLaunchKernelGGL((k_residual<T, TO>), dim3(nblk(total)), dim3(256), 0, s, L, u, f, t, cx, cy, cz);
+}
+
+template <typename T>
+void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s) {
+    long total = (long)Cl.nx * Cl.ny * 2 * Cl.kh;
+    hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(total)), dim3(256), 0, s, F, Cl, t, fc);
+}
+
+template <typename T, typename TF>
+void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s) {
+    long total = (long)F.nx * F.ny * 2 * F.kh;
+    hipLaunchKernelGGL((k_prolong<T, TF>), dim3(nblk(total)), dim3(256), 0, s, F, Cl, e, u, accumulate);
+}
+
+template <typename T>
+void launch_fill(T *a, long n, T v, hipStream_t s) {
+    unsigned b = nblk(n);
+    if (b > 65536u) b = 65536u;
+    if (b == 0) b = 1;
+    hipLaunchKernelGGL(k_fill<T>, dim3(b), dim3(256), 0, s, a, n, v);
+}
+
+template <typename T>
+void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int core, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_scalar<T>, dim3(nblk(lay_size(L))), dim3(256), 0, s, L, a, dv, scale, core);
+}
+
+template <typename TS, typename TD>
+void launch_pack(const Lay &L, const TS *ref, TD *dst, int core_only, hipStream_t s) {
+    long n = (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    hipLaunchKernelGGL((k_pack<TS, TD>), dim3(nblk(n)), dim3(256), 0, s, L, ref, dst, core_only);
+}
+
+template <typename TS, typename TD>
+void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s) {
+    long n = (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    hipLaunchKernelGGL((k_unpack<TS, TD>), dim3(nblk(n)), dim3(256), 0, s, L, src, ref);
+}
+
+int norm_parts(const Lay &L) { return (int)nblk(lay_size(L)); }
+
+template <typename T>
+void launch_norm(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part,
+                 double *out, hipStream_t s) {
+    int nb = norm_parts(L);
+    hipLaunchKernelGGL(k_norm_partial<T>, dim3(nb), dim3(256), 0, s, L, u, f, cx, cy, cz, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
+}
+
+template <typename T>
+void launch_rhs_sums(const Lay &L, const T *f, const double *wx, const double *wy, const double *wz, double *part,
+                     double *out, hipStream_t s) {
+    int nb = norm_parts(L);
+    hipLaunchKernelGGL(k_rhs_sums<T>, dim3(nb), dim3(256), 0, s, L, f, wx, wy, wz, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
+}
+
+template <typename T>
+size_t coarsest_lds_bytes(const Lay &L) {
+    return sizeof(T) * (2 * (size_t)lay_size(L) + 4 * (size_t)(L.nx + L.ny + L.nz + 6));
+}
+
+template <typename T>
+void launch_coarsest(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, const Bc &bc,
+                     double tol, int max_count, int *iters, hipStream_t s) {
+    size_t lds = coarsest_lds_bytes<T>(L);
+    hipLaunchKernelGGL(k_coarsest<T>, dim3(1), dim3(256), lds, s, L, u, f, cx, cy, cz, bc, tol, max_count, iters);
+}
+
+template <typename T>
+void launch_jacobi(const Lay &L, const T *u, const T *f, T *t, const T *cx, const T *cy, const T *cz,
+                   hipStream_t s) {
+    long total = (long)L.nx * L.ny * 2 * L.kh;
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(total)), dim3(256), 0, s, L, u, f, t, cx, cy, cz);
+}
+
+template <typename T>
+void launch_project_small(const Lay &L, T *f, const double *wx, const double *wy, const double *wz, double wsum,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_project_small<T>, dim3(1), dim3(256), 0, s, L, f, wx, wy, wz, wsum);
+}
+
+// explicit instantiations
+#define SR_INST(T)                                                                                          \
+    template void launch_fill_ghosts<T>(const Lay &, T *, const Bc &, hipStream_t);                        \
+    template void launch_rb_sweep<T>(const Lay &, T *, const T *, const T *, const T *, const T *, int,     \
+                                     const Bc &, bool, hipStream_t);                                        \
+    template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t);               \
+    template void launch_fill<T>(T *, long, T, hipStream_t);                                               \
+    template void launch_add_scalar<T>(const Lay &, T *, const double *, double, int, hipStream_t);        \
+    template void launch_norm<T>(const Lay &, const T *, const T *, const T *, const T *, const T *,       \
+                                 double *, double *, hipStream_t);                                          \
+    template void launch_rhs_sums<T>(const Lay &, const T *, const double *, const double *, const double *, \
+                                     double *, double *, hipStream_t);                                      \
+    template size_t coarsest_lds_bytes<T>(const Lay &);                                                     \
+    template void launch_coarsest<T>(const Lay &, T *, const T *, const T *, const T *, const T *,         \
+                                     const Bc &, double, int, int *, hipStream_t);            \
+    template void launch_jacobi<T>(const Lay &, const T *, const T *, T *, const T *, const T *, const T *,  \
+                                   hipStream_t);                                                            \
+    template void launch_project_small<T>(const Lay &, T *, const double *, const double *, const double *, \
+                                          double, hipStream_t);
+SR_INST(double)
+SR_INST(float)
+#undef SR_INST
+
+template void launch_residual<double, double>(const Lay &, const double *, const double *, double *,
+                                              const double *, const double *, const double *, hipStream_t);
+template void launch_residual<float, float>(const Lay &, const float *, const float *, float *, const float *,
+                                            const float *, const float *, hipStream_t);
+template void launch_residual<double, float>(const Lay &, const double *, const double *, float *,
+                                             const double *, const double *, const double *, hipStream_t);
+template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t);
+template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t);
+template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t);
+template void launch_pack<double, double>(const Lay &, const double *, double *, int, hipStream_t);
+template void launch_pack<double, float>(const Lay &, const double *, float *, int, hipStream_t);
+template void launch_unpack<double, double>(const Lay &, const double *, double *, hipStream_t);
+template void launch_unpack<float, double>(const Lay &, const float *, double *, hipStream_t);
+
+}  // namespace sr

@@ -1,0 +1,255 @@
+// Solve drivers of the C ABI: poisson::mgSolve (lib/poisson/poisson.cc:100-151) on the device,
+// in FP64 and in mixed precision, plus the kernel-timing entry points used for roofline figures.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/saras_mg.h"
+#include "mg_kernels.h"
+#include "mg_state.h"
+
+using namespace sr;
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess)                                                                         \
+            return set_err(SR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+#define RETCHK(expr)                                                                                  \
+    do {                                                                                              \
+        int _r = (expr);                                                                              \
+        if (_r != SR_OK) return _r;                                                                   \
+    } while (0)
+
+static bool is_singular(const sr_mg *m) {
+    return !(m->d.testPoisson && !(m->d.xPer && m->d.yPer && m->d.zPer));
+}
+
+static Bc pads_rules(const sr_mg *m) {
+    Bc b{};
+    b.rule[0] = m->d.xPer ? G_PERIODIC : G_KEEP;
+    b.rule[1] = m->d.yPer ? G_PERIODIC : G_KEEP;
+    b.rule[2] = G_KEEP;
+    return b;
+}
+
+static int stage_alloc(sr_mg *m) {
+    if (m->stage) return SR_OK;
+    const Lay &L = m->lv[0].L;
+    size_t n = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    HIPCHK(hipMalloc((void **)&m->stage, sizeof(double) * n));
+    HIPCHK(hipMalloc((void **)&m->stage2, sizeof(double) * n));
+    return SR_OK;
+}
+
+// Loads the right-hand side and the initial guess into level 0 and applies the solvability step:
+// reference semantics subtract the mean of the padded RHS (poisson.cc:120-129); projected
+// semantics remove the component along the left null vector of L.
+static void load_level0(sr_mg *m, const double *drhs, const double *dp) {
+    Level &V = m->lv[0];
+    hipStream_t s = m->stream;
+    long sz = lay_size(V.L);
+    launch_fill<double>(V.u, sz, 0.0, s);
+    launch_fill<double>(V.f, sz, 0.0, s);
+    launch_pack<double, double>(V.L, drhs, V.f, 1, s);
+    launch_pack<double, double>(V.L, dp, V.u, 1, s);
+    bool proj = m->d.semantics == SR_SEM_PROJECTED && is_singular(m);
+    if (proj) {
+        launch_rhs_sums<double>(V.L, V.f, m->wvec[0], m->wvec[1], m->wvec[2], m->part, m->dres, s);
+        launch_add_scalar<double>(V.L, V.f, m->dres, -1.0 / m->wsum, 1, s);
+    }
+    launch_fill_ghosts<double>(V.L, V.f, pads_rules(m), s);
+    launch_fill_ghosts<double>(V.L, V.u, pads_rules(m), s);
+    if (m->d.semantics == SR_SEM_REFERENCE && !m->d.testPoisson) {
+        double npad = (double)(V.L.nx + 2) * (V.L.ny + 2) * (V.L.nz + 2);
+        launch_rhs_sums<double>(V.L, V.f, nullptr, nullptr, nullptr, m->part, m->dres, s);
+        launch_add_scalar<double>(V.L, V.f, m->dres + 1, -1.0 / npad, 0, s);
+    }
+}
+
+static int norm_order(const sr_mg *m) {
+    return m->d.semantics == SR_SEM_PROJECTED ? 2 : m->d.resType;
+}
+
+// one FP64 cycle plus its norm and the read-back of the reduced quantities
+static void cycle_fp64(sr_mg *m) {
+    vcycle_fp64(m);
+    norm_level0(m);
+    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+}
+
+static bool graph_ok(const sr_mg *m) {
+    // Jacobi swaps the level handles; the graph replays fixed pointers, so odd counts run eagerly
+    return m->d.gsSmooth || ((m->d.preSmooth % 2 == 0) && (m->d.postSmooth % 2 == 0));
+}
+
+static int run_cycle(sr_mg *m, int kind, void (*body)(sr_mg *)) {
+    if (!graph_ok(m)) {
+        body(m);
+        HIPCHK(hipGetLastError());
+        return SR_OK;
+    }
+    if (m->graph_kind != kind) {
+        if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
+        if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+        HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        body(m);
+        HIPCHK(hipStreamEndCapture(m->stream, &m->graph));
+        HIPCHK(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0));
+        m->graph_kind = kind;
+    }
+    HIPCHK(hipGraphLaunch(m->graph_exec, m->stream));
+    return SR_OK;
+}
+
+static int solve_fp64(sr_mg *m, int &cycles, double &res, bool &conv, double &sweep_ms) {
+    int nt = norm_order(m);
+    double prev = 0.0;
+    conv = false;
+    double tsum = 0.0;
+    int tcount = 0;
+    for (int it = 0; it < m->d.vcCount; it++) {
+        RETCHK(run_cycle(m, 0, cycle_fp64));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        if (m->d.gsSmooth && m->d.preSmooth > 0) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
+                tsum += ms;
+                tcount += 2 * m->d.preSmooth;
+            }
+        }
+        double r = error_from(m, m->hres, nt);
+        m->history.push_back(r);
+        cycles = it + 1;
+        res = r;
+        if (m->d.semantics == SR_SEM_PROJECTED) {
+            if (r < m->d.relTolerance) { conv = true; break; }
+        } else if (!m->d.testPoisson) {
+            if (std::fabs(prev - r) < m->d.mgTolerance) { conv = true; break; }
+            prev = r;
+        }
+    }
+    if (m->d.testPoisson && m->d.semantics != SR_SEM_PROJECTED) conv = true;   // fixed cycle count
+    sweep_ms = tcount ? tsum / tcount : 0.0;
+    return SR_OK;
+}
+
+extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device, sr_mg_stats *st) {
+    if (!m || !p || !rhs) return set_err(SR_ERR_ARG, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    const Lay &L = m->lv[0].L;
+    size_t n = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    const double *drhs = rhs;
+    double *dp = p;
+    if (!on_device) {
+        RETCHK(stage_alloc(m));
+        HIPCHK(hipMemcpyAsync(m->stage, rhs, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(m->stage2, p, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+        drhs = m->stage;
+        dp = m->stage2;
+    }
+    m->history.clear();
+    m->vLevel = 0;
+    load_level0(m, drhs, dp);
+    HIPCHK(hipGetLastError());
+    int cycles = 0;
+    double res = 0.0;
+    bool conv = false;
+    double sweep_ms = 0.0;
+    if (m->d.precision == SR_PREC_FP64) {
+        RETCHK(solve_fp64(m, cycles, res, conv, sweep_ms));
+    } else {
+        return set_err(SR_ERR_ARG, "precision mode not available");
+    }
+    // closing ghosts of the last post-smooth (poisson_d3.cc:181) and the copy over the padded inFn
+    // (poisson.cc:151)
+    Level &V = m->lv[0];
+    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), m->stream);
+    launch_unpack<double, double>(V.L, V.u, dp, m->stream);
+    if (!on_device) HIPCHK(hipMemcpyAsync(p, dp, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipGetLastError());
+    m->vLevel = 0;
+    m->zeroBC = 0;
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->cycles = cycles;
+        st->converged = conv ? 1 : 0;
+        st->residual = res;
+        st->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->sweep0_ms = sweep_ms;
+    }
+    if (!conv && m->d.semantics == SR_SEM_PROJECTED)
+        return set_err(SR_ERR_NOT_CONVERGED, "relative residual above relTolerance after vcCount cycles");
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// measurement
+// ------------------------------------------------------------------------------------------------
+extern "C" int sr_mg_kernel_bytes(const sr_mg *m, int kernel, int level, double *bytes) {
+    if (!m || !bytes || level < 0 || level > m->depth) return set_err(SR_ERR_ARG, "bad arguments");
+    const Lay &L = m->lv[level].L;
+    double nn = (double)L.nx * L.ny * L.nz;
+    double w = 8.0;
+    switch (kernel) {
+        case 0: *bytes = 12.0 * nn / 8.0 * w; break;              // half sweep: u(other), f, u(this) halves
+        case 1: *bytes = 3.0 * nn * w; break;                     // residual: read u, f; write t
+        case 2: {                                                  // restriction: read fine t, write coarse f
+            if (level >= m->depth) return set_err(SR_ERR_ARG, "no coarser level");
+            const Lay &Cl = m->lv[level + 1].L;
+            *bytes = (nn + (double)Cl.nx * Cl.ny * Cl.nz) * w;
+            break;
+        }
+        case 3: {                                                  // prolongation-correction into `level`
+            if (level >= m->depth) return set_err(SR_ERR_ARG, "no coarser level");
+            const Lay &Cl = m->lv[level + 1].L;
+            *bytes = (2.0 * nn + (double)Cl.nx * Cl.ny * Cl.nz) * w;
+            break;
+        }
+        case 4: {
+            double g = (double)(L.nx + 2) * (L.ny + 2) * (L.nz + 2) - nn;
+            *bytes = 2.0 * g * w;
+            break;
+        }
+        case 5: *bytes = 2.0 * nn * w; break;                     // norm: read u, f
+        default: return set_err(SR_ERR_ARG, "unknown kernel id");
+    }
+    return SR_OK;
+}
+
+extern "C" int sr_mg_time_kernel(sr_mg *m, int kernel, int level, int reps, double *avg_ms) {
+    if (!m || !avg_ms || reps < 1 || level < 0 || level > m->depth) return set_err(SR_ERR_ARG, "bad arguments");
+    if ((kernel == 2 || kernel == 3) && level >= m->depth) return set_err(SR_ERR_ARG, "no coarser level");
+    Level &V = m->lv[level];
+    hipStream_t s = m->stream;
+    Bc bc = pressure_bc(m, level, 1);
+    auto launch = [&](int r) {
+        switch (kernel) {
+            case 0: launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], r & 1, bc, false, s); break;
+            case 1: launch_residual<double, double>(V.L, V.u, V.f, V.t, V.c64[0], V.c64[1], V.c64[2], s); break;
+            case 2: launch_restrict<double>(V.L, m->lv[level + 1].L, V.t, m->lv[level + 1].f, s); break;
+            case 3: launch_prolong<double, double>(V.L, m->lv[level + 1].L, m->lv[level + 1].u, V.u, 1, s); break;
+            case 4: launch_fill_ghosts<double>(V.L, V.u, bc, s); break;
+            case 5: launch_norm<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part, m->dres, s); break;
+        }
+    };
+    if (kernel < 0 || kernel > 5) return set_err(SR_ERR_ARG, "unknown kernel id");
+    launch(0);
+    launch(1);
+    HIPCHK(hipEventRecord(m->ev0, s));
+    for (int r = 0; r < reps; r++) launch(r);
+    HIPCHK(hipEventRecord(m->ev1, s));
+    HIPCHK(hipEventSynchronize(m->ev1));
+    HIPCHK(hipGetLastError());
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    *avg_ms = (double)ms / reps;
+    return SR_OK;
+}

@@ -1,0 +1,75 @@
+// Internal state of a solver handle (include/saras_mg.h) and the operators shared between the
+// per-operator entry points (mg_api.hip) and the solve drivers (mg_solve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/saras_mg.h"
+#include "mg_layout.h"
+
+namespace sr {
+
+struct Level {
+    Lay L{};
+    double h[3] = {0, 0, 0};              // hx(l), hy(l), hz(l)
+    std::vector<double> coef_h[3];        // host copy: per padded index {lp, lm, lc, 0}
+    double *c64[3] = {nullptr, nullptr, nullptr};
+    float *c32[3] = {nullptr, nullptr, nullptr};
+    double *u = nullptr, *f = nullptr, *t = nullptr;        // FP64 pressureData / residualData / tmp
+    float *u32 = nullptr, *f32 = nullptr, *t32 = nullptr;   // FP32 hierarchy (mixed precision)
+    double *wc[3] = {nullptr, nullptr, nullptr};            // null vectors of this level (coarsest)
+    double wcsum = 0;
+};
+
+}  // namespace sr
+
+struct sr_mg {
+    sr_mg_desc d{};
+    int depth = 0;
+    int vLevel = 0;
+    int zeroBC = 0;
+    int maxCount = 0;
+    std::vector<sr::Level> lv;
+    std::vector<double> met0[6];
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t sw0 = nullptr, sw1 = nullptr;   // around the finest pre-smoothing sweeps of a cycle
+    double *wall[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *wvec[3] = {nullptr, nullptr, nullptr};
+    double wsum = 0;
+    double *part = nullptr;      // reduction partials
+    double *dres = nullptr;      // reduced quantities (device)
+    double *hres = nullptr;      // pinned host mirror
+    int *diters = nullptr;
+    double *stage = nullptr, *stage2 = nullptr;   // reference-layout staging (finest level)
+    float *r32 = nullptr;        // FP32 finest residual (mixed precision)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_kind = -1;
+    std::vector<double> history;
+};
+
+namespace sr {
+
+int set_err(int code, const std::string &msg);
+Bc pressure_bc(const sr_mg *m, int level, int zeroBC);
+Bc tmp_bc(const sr_mg *m);
+void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
+                  hipEvent_t ea = nullptr, hipEvent_t eb = nullptr);
+void residual_level(sr_mg *m, int l);
+void coarsest_level(sr_mg *m, int l, int zeroBC);
+void vcycle_fp64(sr_mg *m);
+void norm_level0(sr_mg *m);
+double error_from(const sr_mg *m, const double *q, int norm);
+
+template <typename T>
+void launch_jacobi(const Lay &L, const T *u, const T *f, T *t, const T *cx, const T *cy, const T *cz,
+                   hipStream_t s);
+template <typename T>
+void launch_project_small(const Lay &L, T *f, const double *wx, const double *wy, const double *wz, double wsum,
+                          hipStream_t s);
+
+}  // namespace sr
