@@ -154,6 +154,7 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
             RETCHK(upload_vec(&m->wall[q], w));
         }
     }
+    if (d->precision == SR_PREC_MIXED) RETCHK(ensure_fp32(m));
     // null vectors for the range projection (finest level)
     const Level &V0 = m->lv[0];
     int per[3] = {d->xPer, d->yPer, d->zPer};
@@ -215,6 +216,10 @@ extern "C" int sr_mg_destroy(sr_mg *m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
+    for (int q = 0; q < 2; q++) {
+        if (m->graph_mixed[q]) (void)hipGraphExecDestroy(m->graph_mixed[q]);
+        if (m->graph_mixed_g[q]) (void)hipGraphDestroy(m->graph_mixed_g[q]);
+    }
     for (Level &V : m->lv) {
         (void)hipFree(V.u); (void)hipFree(V.f); (void)hipFree(V.t);
         (void)hipFree(V.u32); (void)hipFree(V.f32); (void)hipFree(V.t32);
@@ -282,6 +287,19 @@ static bool singular(const sr_mg *m) {
 // ------------------------------------------------------------------------------------------------
 // operators (device, FP64 hierarchy)
 // ------------------------------------------------------------------------------------------------
+// Timing marks inside a captured cycle must be external event nodes; a runtime that refuses them
+// loses the in-solve timing, not the solve.
+void sr::mark(sr_mg *m, hipEvent_t e) {
+    if (!e || m->marks_off) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(m->stream, &cs);
+    hipError_t r = cs == hipStreamCaptureStatusActive ? hipEventRecordWithFlags(e, m->stream, hipEventRecordExternal)
+                                                      : hipEventRecord(e, m->stream);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        m->marks_off = true;
+    }
+}
 // smooth(): ghosts, then `count` red-black sweeps (each updated node refreshes its face ghosts);
 // exact = 1 adds the closing imposeBC of poisson_d3.cc:181 (edge and corner ghosts included).
 void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact, hipEvent_t ea,
@@ -291,12 +309,12 @@ void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, b
     hipStream_t s = m->stream;
     if (m->d.gsSmooth) {
         if (!zero_start) launch_fill_ghosts<double>(V.L, V.u, bc, s);
-        if (ea) (void)hipEventRecord(ea, s);
+        if (ea) mark(m, ea);
         for (int n = 0; n < count; n++) {
             launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 0, bc, zero_start && n == 0, s);
             launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 1, bc, false, s);
         }
-        if (eb) (void)hipEventRecord(eb, s);
+        if (eb) mark(m, eb);
     } else {
         // Jacobi: tmp(l) = 0; per sweep {imposeBC; tmp = J(u); swap hierarchies} (poisson_d3.cc:152-168)
         launch_fill<double>(V.t, lay_size(V.L), 0.0, s);
@@ -352,6 +370,77 @@ void sr::vcycle_fp64(sr_mg *m) {
         launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream);
         smooth_level(m, l, d.postSmooth, l == 0 ? 0 : 1, false, false);
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP32 hierarchy (mixed precision): the V-cycle on the error equation A e = r with e = 0 at start.
+// Level-0 ghosts follow the BCs of the error: periodic / mirror as for u, Dirichlet walls -> 0.
+// ------------------------------------------------------------------------------------------------
+static Bc error_bc(const sr_mg *m, int l, int zeroBC) {
+    Bc b = pressure_bc(m, l, zeroBC);
+    for (int q = 0; q < 6; q++) b.wall[q] = nullptr;
+    return b;
+}
+
+static void smooth32(sr_mg *m, int l, int count, int zeroBC, bool zero_start, hipEvent_t ea = nullptr,
+                     hipEvent_t eb = nullptr) {
+    Level &V = m->lv[l];
+    Bc bc = error_bc(m, l, zeroBC);
+    bool wall = bc.rule[0] == G_WALL || bc.rule[1] == G_WALL || bc.rule[2] == G_WALL;
+    bool zs = zero_start && !wall;
+    hipStream_t s = m->stream;
+    if (!zs) {
+        if (zero_start) launch_fill<float>(V.u32, lay_size(V.L), 0.0f, s);
+        launch_fill_ghosts<float>(V.L, V.u32, bc, s);
+    }
+    if (ea) mark(m, ea);
+    for (int n = 0; n < count; n++) {
+        launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], 0, bc, zs && n == 0, s);
+        launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], 1, bc, false, s);
+    }
+    if (eb) mark(m, eb);
+}
+
+void sr::vcycle_fp32(sr_mg *m) {
+    const sr_mg_desc &d = m->d;
+    int D = m->depth;
+    hipStream_t s = m->stream;
+    smooth32(m, 0, d.preSmooth, 0, true, m->sw0, m->sw1);
+    for (int l = 0; l < D; l++) {
+        Level &V = m->lv[l];
+        launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s);
+        launch_fill_ghosts<float>(V.L, V.t32, tmp_bc(m), s);
+        launch_restrict<float>(V.L, m->lv[l + 1].L, V.t32, m->lv[l + 1].f32, s);
+        if (l + 1 == D) {
+            Level &Cv = m->lv[D];
+            launch_fill<float>(Cv.u32, lay_size(Cv.L), 0.0f, s);
+            if (d.coarseProject && singular(m))
+                launch_project_small<float>(Cv.L, Cv.f32, Cv.wc[0], Cv.wc[1], Cv.wc[2], Cv.wcsum, s);
+            launch_coarsest<float>(Cv.L, Cv.u32, Cv.f32, Cv.c32[0], Cv.c32[1], Cv.c32[2], error_bc(m, D, 1),
+                                   d.mgTolerance, m->maxCount, m->diters, s);
+        } else {
+            smooth32(m, l + 1, d.preSmooth, 1, true);
+        }
+    }
+    for (int l = D - 1; l >= 0; l--) {
+        launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s);
+        smooth32(m, l, d.postSmooth, l == 0 ? 0 : 1, false);
+    }
+}
+
+int sr::ensure_fp32(sr_mg *m) {
+    for (Level &V : m->lv) {
+        if (V.u32) continue;
+        long sz = lay_size(V.L);
+        HIPCHK(hipMalloc((void **)&V.u32, sizeof(float) * sz));
+        HIPCHK(hipMalloc((void **)&V.f32, sizeof(float) * sz));
+        HIPCHK(hipMalloc((void **)&V.t32, sizeof(float) * sz));
+        launch_fill<float>(V.u32, sz, 0.0f, m->stream);
+        launch_fill<float>(V.f32, sz, 0.0f, m->stream);
+        launch_fill<float>(V.t32, sz, 0.0f, m->stream);
+    }
+    if (coarsest_lds_bytes<float>(m->lv[m->depth].L) > 60 * 1024) return set_err(SR_ERR_ARG, "coarsest level too large");
+    return SR_OK;
 }
 
 void sr::norm_level0(sr_mg *m) {

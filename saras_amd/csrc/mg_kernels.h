@@ -33,6 +33,11 @@ template <typename T>
 void launch_rhs_sums(const Lay &L, const T *f, const double *wx, const double *wy, const double *wz, double *part,
                      double *out, hipStream_t s);
 template <typename T> size_t coarsest_lds_bytes(const Lay &L);
+// iterative-refinement step: e == nullptr computes r = f - L u_in only; otherwise u_out = u_in + e
+// (core and face ghosts) and r = f - L u_out.  r is stored in FP32; out[0..3] as launch_norm.
+void launch_outer_residual(const Lay &L, const double *u_in, const float *e, const double *f, double *u_out,
+                           float *r32, const double *cx, const double *cy, const double *cz, const Bc &bc,
+                           double *part, double *out, hipStream_t s);
 template <typename T>
 void launch_coarsest(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, const Bc &bc,
                      double tol, int max_count, int *iters, hipStream_t s);

@@ -602,6 +602,59 @@ __global__ void __launch_bounds__(256) k_project_small(Lay L, T *__restrict__ f,
 }
 
 // ------------------------------------------------------------------------------------------------
+// mixed precision: outer FP64 step of iterative refinement
+//   CORR: u_out = u_in + e, r = f - L u_out (u_out read on the fly from u_in + e at the stencil)
+//   else: u_out untouched, r = f - L u_in
+// r goes to the FP32 right-hand side of the inner cycle; the block partials carry
+// sum r^2, sum f^2 (core), max |r| (core), max |f| (every padded node) in FP64.
+// ------------------------------------------------------------------------------------------------
+template <bool CORR>
+__global__ void __launch_bounds__(256) k_outer_residual(Lay L, const double *__restrict__ u_in,
+                                                        const float *__restrict__ e, const double *__restrict__ f,
+                                                        double *__restrict__ u_out, float *__restrict__ r32,
+                                                        const double *__restrict__ cx, const double *__restrict__ cy,
+                                                        const double *__restrict__ cz, Bc bc,
+                                                        double *__restrict__ part) {
+    long t = (long)xcd_block() * 256 + threadIdx.x;
+    long n = lay_size(L);
+    double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
+    if (t < n) {
+        int h = (int)(t % L.khs);
+        long rr = t / L.khs;
+        int c = (int)(rr & 1);
+        rr >>= 1;
+        int jp = (int)(rr % (L.ny + 2));
+        int ip = (int)(rr / (L.ny + 2));
+        int kb = (c + 1 + ip + jp) & 1;
+        int kp = 2 * h + kb;
+        if (h < L.kh && kp <= L.nz + 1) {
+            double fv = f[t];
+            m3 = fabs(fv);
+            if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
+                long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+                long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
+                auto U = [&](long q) { return CORR ? u_in[q] + (double)e[q] : u_in[q]; };
+                double uc = U(rc + h);
+                const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+                double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) +
+                            Y[1] * U(ro - sy + h) + Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) -
+                            (X[2] + Y[2] + Z[2]) * uc;
+                double r = fv - lu;
+                r32[t] = (float)r;
+                if (CORR) {
+                    u_out[t] = uc;
+                    write_images(L, u_out, ip, jp, kp, uc, bc);
+                }
+                s0 = r * r;
+                s1 = fv * fv;
+                m2 = fabs(r);
+            }
+        }
+    }
+    block_reduce4(s0, s1, m2, m3, part);
+}
+
+// ------------------------------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------------------------------
 static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -707,6 +760,19 @@ template <typename T>
 void launch_project_small(const Lay &L, T *f, const double *wx, const double *wy, const double *wz, double wsum,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_project_small<T>, dim3(1), dim3(256), 0, s, L, f, wx, wy, wz, wsum);
+}
+
+void launch_outer_residual(const Lay &L, const double *u_in, const float *e, const double *f, double *u_out,
+                           float *r32, const double *cx, const double *cy, const double *cz, const Bc &bc,
+                           double *part, double *out, hipStream_t s) {
+    int nb = norm_parts(L);
+    if (e)
+        hipLaunchKernelGGL(k_outer_residual<true>, dim3(nb), dim3(256), 0, s, L, u_in, e, f, u_out, r32, cx, cy, cz,
+                           bc, part);
+    else
+        hipLaunchKernelGGL(k_outer_residual<false>, dim3(nb), dim3(256), 0, s, L, u_in, e, f, u_out, r32, cx, cy,
+                           cz, bc, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 
 // explicit instantiations

@@ -89,8 +89,29 @@ static bool graph_ok(const sr_mg *m) {
     return m->d.gsSmooth || ((m->d.preSmooth % 2 == 0) && (m->d.postSmooth % 2 == 0));
 }
 
+// Capture `body` into an executable graph.  A capture the runtime rejects is retried once without
+// the timing marks; if that fails too the cycle runs eagerly from then on.
+static bool capture(sr_mg *m, void (*body)(sr_mg *), hipGraph_t &g, hipGraphExec_t &ex) {
+    for (int attempt = 0; attempt < 2; attempt++) {
+        g = nullptr;
+        ex = nullptr;
+        if (hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) break;
+        body(m);
+        hipError_t e1 = hipStreamEndCapture(m->stream, &g);
+        if (e1 == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) return true;
+        (void)hipGetLastError();
+        if (g) (void)hipGraphDestroy(g);
+        g = nullptr;
+        ex = nullptr;
+        m->marks_off = true;
+    }
+    (void)hipGetLastError();
+    m->graphs_off = true;
+    return false;
+}
+
 static int run_cycle(sr_mg *m, int kind, void (*body)(sr_mg *)) {
-    if (!graph_ok(m)) {
+    if (!graph_ok(m) || m->graphs_off) {
         body(m);
         HIPCHK(hipGetLastError());
         return SR_OK;
@@ -98,10 +119,11 @@ static int run_cycle(sr_mg *m, int kind, void (*body)(sr_mg *)) {
     if (m->graph_kind != kind) {
         if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
         if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
-        HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-        body(m);
-        HIPCHK(hipStreamEndCapture(m->stream, &m->graph));
-        HIPCHK(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0));
+        if (!capture(m, body, m->graph, m->graph_exec)) {
+            body(m);
+            HIPCHK(hipGetLastError());
+            return SR_OK;
+        }
         m->graph_kind = kind;
     }
     HIPCHK(hipGraphLaunch(m->graph_exec, m->stream));
@@ -140,6 +162,86 @@ static int solve_fp64(sr_mg *m, int &cycles, double &res, bool &conv, double &sw
     return SR_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// mixed precision: u (FP64, two ping-pong buffers: level-0 u and the unused level-0 tmp) is
+// corrected by FP32 V-cycles on the error equation; residual, correction and norm stay FP64
+// ------------------------------------------------------------------------------------------------
+
+static void mixed_body_ab(sr_mg *m) {
+    Level &V = m->lv[0];
+    vcycle_fp32(m);
+    launch_outer_residual(V.L, V.u, V.u32, V.f, V.t, V.f32, V.c64[0], V.c64[1], V.c64[2], pressure_bc(m, 0, 0),
+                          m->part, m->dres, m->stream);
+    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+}
+
+static void mixed_body_ba(sr_mg *m) {
+    Level &V = m->lv[0];
+    vcycle_fp32(m);
+    launch_outer_residual(V.L, V.t, V.u32, V.f, V.u, V.f32, V.c64[0], V.c64[1], V.c64[2], pressure_bc(m, 0, 0),
+                          m->part, m->dres, m->stream);
+    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+}
+
+static int run_graph2(sr_mg *m, int which) {
+    void (*body)(sr_mg *) = which == 0 ? mixed_body_ab : mixed_body_ba;
+    hipGraphExec_t &ex = m->graph_mixed[which];
+    if (!ex && !m->graphs_off) capture(m, body, m->graph_mixed_g[which], ex);
+    if (!ex) {
+        body(m);
+        HIPCHK(hipGetLastError());
+        return SR_OK;
+    }
+    HIPCHK(hipGraphLaunch(ex, m->stream));
+    return SR_OK;
+}
+
+static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &sweep_ms, double *&dsol) {
+    if (!m->d.gsSmooth) return set_err(SR_ERR_ARG, "mixed precision needs gsSmooth = 1");
+    RETCHK(ensure_fp32(m));
+    Level &V = m->lv[0];
+    long sz = lay_size(V.L);
+    HIPCHK(hipMemcpyAsync(V.t, V.u, sizeof(double) * sz, hipMemcpyDeviceToDevice, m->stream));
+    launch_outer_residual(V.L, V.u, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
+                          pressure_bc(m, 0, 0), m->part, m->dres, m->stream);
+    HIPCHK(hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    int nt = norm_order(m);
+    double r = error_from(m, m->hres, nt);
+    dsol = V.u;
+    conv = false;
+    cycles = 0;
+    res = r;
+    if (m->d.semantics == SR_SEM_PROJECTED && r < m->d.relTolerance) { conv = true; return SR_OK; }
+    double prev = 0.0, tsum = 0.0;
+    int tcount = 0;
+    int cur = 0;
+    for (int it = 0; it < m->d.vcCount; it++) {
+        RETCHK(run_graph2(m, cur));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        float ms = 0.f;
+        if (m->d.preSmooth > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
+            tsum += ms;
+            tcount += 2 * m->d.preSmooth;
+        }
+        cur ^= 1;
+        dsol = cur ? V.t : V.u;
+        r = error_from(m, m->hres, nt);
+        m->history.push_back(r);
+        cycles = it + 1;
+        res = r;
+        if (m->d.semantics == SR_SEM_PROJECTED) {
+            if (r < m->d.relTolerance) { conv = true; break; }
+        } else if (!m->d.testPoisson) {
+            if (std::fabs(prev - r) < m->d.mgTolerance) { conv = true; break; }
+            prev = r;
+        }
+    }
+    if (m->d.testPoisson && m->d.semantics != SR_SEM_PROJECTED) conv = true;
+    sweep_ms = tcount ? tsum / tcount : 0.0;
+    return SR_OK;
+}
+
 extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device, sr_mg_stats *st) {
     if (!m || !p || !rhs) return set_err(SR_ERR_ARG, "null argument");
     auto t0 = std::chrono::steady_clock::now();
@@ -162,16 +264,19 @@ extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device
     double res = 0.0;
     bool conv = false;
     double sweep_ms = 0.0;
+    Level &V = m->lv[0];
+    double *dsol = V.u;
     if (m->d.precision == SR_PREC_FP64) {
         RETCHK(solve_fp64(m, cycles, res, conv, sweep_ms));
+    } else if (m->d.precision == SR_PREC_MIXED) {
+        RETCHK(solve_mixed(m, cycles, res, conv, sweep_ms, dsol));
     } else {
-        return set_err(SR_ERR_ARG, "precision mode not available");
+        return set_err(SR_ERR_ARG, "unknown precision mode");
     }
     // closing ghosts of the last post-smooth (poisson_d3.cc:181) and the copy over the padded inFn
     // (poisson.cc:151)
-    Level &V = m->lv[0];
-    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), m->stream);
-    launch_unpack<double, double>(V.L, V.u, dp, m->stream);
+    launch_fill_ghosts<double>(V.L, dsol, pressure_bc(m, 0, 0), m->stream);
+    launch_unpack<double, double>(V.L, dsol, dp, m->stream);
     if (!on_device) HIPCHK(hipMemcpyAsync(p, dp, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     HIPCHK(hipGetLastError());

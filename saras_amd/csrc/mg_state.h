@@ -37,6 +37,8 @@ struct sr_mg {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t sw0 = nullptr, sw1 = nullptr;   // around the finest pre-smoothing sweeps of a cycle
+    bool marks_off = false;
+    bool graphs_off = false;
     double *wall[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *wvec[3] = {nullptr, nullptr, nullptr};
     double wsum = 0;
@@ -49,12 +51,15 @@ struct sr_mg {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_kind = -1;
+    hipGraph_t graph_mixed_g[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_mixed[2] = {nullptr, nullptr};
     std::vector<double> history;
 };
 
 namespace sr {
 
 int set_err(int code, const std::string &msg);
+void mark(sr_mg *m, hipEvent_t e);
 Bc pressure_bc(const sr_mg *m, int level, int zeroBC);
 Bc tmp_bc(const sr_mg *m);
 void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
@@ -62,6 +67,8 @@ void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool 
 void residual_level(sr_mg *m, int l);
 void coarsest_level(sr_mg *m, int l, int zeroBC);
 void vcycle_fp64(sr_mg *m);
+void vcycle_fp32(sr_mg *m);
+int ensure_fp32(sr_mg *m);
 void norm_level0(sr_mg *m);
 double error_from(const sr_mg *m, const double *q, int norm);
 

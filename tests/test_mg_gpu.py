@@ -258,3 +258,48 @@ def test_257_cubed_projected_solve_reaches_tolerance():
     o.array(po.R, 0)[...] = fp
     r = o.compute_error(2)
     assert r < 1.2e-10, r
+
+
+# ------------------------------------------------------------------------------------------------
+# mixed precision: FP32 inner V-cycles under FP64 iterative refinement
+# ------------------------------------------------------------------------------------------------
+def _mixed_pair(ind, mesh="UUD", domain="PPN", **kw):
+    from saras_amd import _lib
+    pb = Problem(ind=ind, domain=domain, mesh=mesh, semantics=po.SEM_PROJECTED, rel_tol=1e-10, **kw)
+    return pb, pb.gpu(), pb.gpu(precision=_lib.SR_PREC_MIXED)
+
+
+@pytest.mark.parametrize("ind", [(5, 5, 5), (6, 6, 5)])
+def test_mixed_precision_reaches_fp64_solution(ind):
+    pb, g64, gm = _mixed_pair(ind)
+    f = pb.random_rhs(21)
+    p64 = np.zeros(pb.shape)
+    s64 = g64.mgSolve(p64, f)
+    pm = np.zeros(pb.shape)
+    sm = gm.mgSolve(pm, f)
+    assert s64.converged and sm.converged and sm.residual < 1e-10
+    assert abs(sm.cycles - s64.cycles) <= 2
+    # both solve the same singular system: compare up to the constant null vector
+    a, b = pm[1:-1, 1:-1, 1:-1], p64[1:-1, 1:-1, 1:-1]
+    d = (a - a.mean()) - (b - b.mean())
+    assert np.max(np.abs(d)) < 1e-7 * np.max(np.abs(b - b.mean()))
+    # the oracle confirms the mixed solution's FP64 residual
+    o = pb.oracle()
+    o.level = 0
+    o.array(po.P, 0)[...] = pm
+    o.array(po.R, 0)[...] = o.project_rhs(f.copy())
+    assert o.compute_error(2) < 1.2e-10
+
+
+def test_mixed_precision_dirichlet_r2():
+    from saras_amd import _lib, mg
+    n = (33, 33, 33)
+    pr = mg.Parser(xInd=5, yInd=5, zInd=5, vcDepth=4, vcCount=40, testPoisson=True, semantics=_lib.SR_SEM_PROJECTED,
+                   precision=_lib.SR_PREC_MIXED, relTolerance=1e-10)
+    gm = mg.Multigrid(mg.Grid(pr), pr, walls=r2_walls(n))
+    f = np.zeros(tuple(k + 2 for k in n))
+    f[1:-1, 1:-1, 1:-1] = 1.0
+    p = np.zeros_like(f)
+    st = gm.mgSolve(p, f)
+    assert st.converged
+    assert np.max(np.abs(p[1:-1, 1:-1, 1:-1] - r2_exact(n))) < 1e-8
