@@ -58,44 +58,63 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     from saras_amd import mg, workloads
+    from saras_amd._lib import DeviceArray
 
     p, g, f = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)
+    # inputs resident in HBM before the timed region: the right-hand side and a zero initial guess
+    d_rhs, d_p0, d_p = DeviceArray(f.shape), DeviceArray(f.shape), DeviceArray(f.shape)
+    d_rhs.upload(f)
+    d_p0.upload(np.zeros(f.shape))
+
+    def run(solver):
+        def step():
+            # the solve overwrites its in/out argument: start every step from the zero guess
+            mg._L.check(mg._L.load().sr_memcpy(d_p.ptr, d_p0.ptr, d_p.nbytes, 3))
+            return solver.mgSolve_device(d_p.ptr, d_rhs.ptr)
+
+        for _ in range(args.warmup):
+            step()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            stats.append(step())
+        t1 = time.perf_counter()
+        return (t1 - t0) * 1e3 / args.steps, stats
+
+    # plain FP64 multigrid (the reference algorithm, red-black order)
     solver = mg.Multigrid(g, p)
-    dev = torch.device("cuda:0")
-    d_rhs = torch.from_numpy(f).to(dev)
-    d_p = torch.zeros(f.shape, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    def step():
-        d_p.zero_()
-        torch.cuda.synchronize()
-        return solver.mgSolve_device(d_p.data_ptr(), d_rhs.data_ptr())
-
-    for _ in range(args.warmup):
-        st = step()
-    torch.cuda.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(step())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    ms, stats = run(solver)
+    sol64 = d_p.download()
     cycles = [s.cycles for s in stats]
     res = max(s.residual for s in stats)
     conv = all(s.converged for s in stats)
     sweep_ms = float(np.mean([s.sweep0_ms for s in stats]))
-
     # roofline of the dominant kernel: the finest-level red-black half sweep
     bytes_sweep = solver.kernel_bytes(0, 0)
     achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else None
     iso_ms = solver.time_kernel(0, 0, 20)
+    solver.close()
     # whole-cycle algorithmic traffic (130 B per node per level + 16 B per finest node for the norm)
     n_levels = [((1 << (8 - l)) + 1) ** 3 for l in range(p.vcDepth + 1)]
     cyc_bytes = 130.0 * sum(n_levels[:-1]) + 16.0 * n_levels[0]
     solve_gbs = cyc_bytes * np.mean(cycles) / (ms * 1e-3) / 1e9
+
+    # mixed precision: FP32 V(2,2) cycles under FP64 iterative refinement, same stop rule
+    pm = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)[0]
+    pm.precision = mg._L.SR_PREC_MIXED
+    solver_m = mg.Multigrid(g, pm)
+    ms_m, stats_m = run(solver_m)
+    solm = d_p.download()
+    solver_m.close()
+    a, b = solm[1:-1, 1:-1, 1:-1], sol64[1:-1, 1:-1, 1:-1]
+    dm = (a - a.mean()) - (b - b.mean())
+    mixed = {"ms_per_solve": round(ms_m, 4), "cycles": int(np.median([s.cycles for s in stats_m])),
+             "final_rel_residual": max(s.residual for s in stats_m),
+             "converged": all(s.converged for s in stats_m),
+             "max_abs_diff_vs_fp64": float(np.max(np.abs(dm))),
+             "rel_l2_diff_vs_fp64": float(np.linalg.norm(dm) / np.linalg.norm(b - b.mean())),
+             "speedup_vs_fp64": round(ms / ms_m, 3)}
 
     cpu = None
     if not args.no_cpu:
@@ -113,7 +132,7 @@ def main():
                                "V(2,2) RB-GS to 1e-10 rel. residual (plain FP64 multigrid)",
                    "nodes": [257, 257, 257], "vcDepth": p.vcDepth, "variant": "fp64-mg",
                    "cycles_per_solve": int(np.median(cycles)), "final_rel_residual": res, "converged": conv,
-                   "solve_GBs_algorithmic": round(solve_gbs, 1)},
+                   "solve_GBs_algorithmic": round(solve_gbs, 1), "mixed_precision": mixed},
         "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<double> finest level",
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,

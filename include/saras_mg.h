@@ -131,6 +131,14 @@ int sr_mg_kernel_bytes(const sr_mg *mg, int kernel, int level, double *bytes);
 int sr_grid_axis(int n, double len, int grid_type, double beta, int faces, double *x, double *xi_x,
                  double *xixx, double *xix2);
 
+/* ---- device memory ----------------------------------------------------------------------------- */
+/* HBM-resident fields for on_device solves: the device counterpart of the Blitz allocation every
+ * reference field makes (lib/field/field.cc:75-86).  kind: 1 host->device, 2 device->host,
+ * 3 device->device.  Copies complete before return.                                               */
+int sr_device_alloc(unsigned long bytes, void **ptr);
+int sr_device_free(void *ptr);
+int sr_memcpy(void *dst, const void *src, unsigned long bytes, int kind);
+
 /* Last error message of the calling thread (never NULL). */
 const char *sr_last_error(void);
 

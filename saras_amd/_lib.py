@@ -63,6 +63,9 @@ SIGNATURES = [
     ("sr_mg_time_kernel", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     ("sr_mg_kernel_bytes", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     ("sr_grid_axis", C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+    ("sr_device_alloc", C.c_int, [C.c_ulong, C.POINTER(C.c_void_p)]),
+    ("sr_device_free", C.c_int, [C.c_void_p]),
+    ("sr_memcpy", C.c_int, [C.c_void_p, C.c_void_p, C.c_ulong, C.c_int]),
     ("sr_last_error", C.c_char_p, []),
 ]
 
@@ -96,3 +99,38 @@ def check(code):
         msg = load().sr_last_error()
         raise SarasError(code, msg.decode() if msg else "")
     return code
+
+
+class DeviceArray:
+    """A float64 buffer in HBM owned through the C ABI (no torch involved)."""
+
+    def __init__(self, shape):
+        import numpy as np
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = C.c_void_p()
+        check(load().sr_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape
+        check(load().sr_memcpy(self.ptr, a.ctypes.data, self.nbytes, 1))
+
+    def download(self):
+        import numpy as np
+        a = np.empty(self.shape)
+        check(load().sr_memcpy(a.ctypes.data, self.ptr, self.nbytes, 2))
+        return a
+
+    def free(self):
+        if self.ptr:
+            load().sr_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -612,3 +612,25 @@ extern "C" int sr_mg_history(const sr_mg *m, double *out, int cap) {
     for (int i = 0; i < n; i++) out[i] = m->history[i];
     return n;
 }
+
+// ------------------------------------------------------------------------------------------------
+// device memory helpers
+// ------------------------------------------------------------------------------------------------
+extern "C" int sr_device_alloc(unsigned long bytes, void **ptr) {
+    if (!ptr || bytes == 0) return set_err(SR_ERR_ARG, "bad allocation request");
+    HIPCHK(hipMalloc(ptr, bytes));
+    return SR_OK;
+}
+
+extern "C" int sr_device_free(void *ptr) {
+    if (ptr) HIPCHK(hipFree(ptr));
+    return SR_OK;
+}
+
+extern "C" int sr_memcpy(void *dst, const void *src, unsigned long bytes, int kind) {
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                    : kind == 3 ? hipMemcpyDeviceToDevice : hipMemcpyDefault;
+    if (kind < 1 || kind > 3 || !dst || !src) return set_err(SR_ERR_ARG, "bad copy request");
+    HIPCHK(hipMemcpy(dst, src, bytes, k));
+    return SR_OK;
+}

@@ -201,6 +201,9 @@ static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &s
     RETCHK(ensure_fp32(m));
     Level &V = m->lv[0];
     long sz = lay_size(V.L);
+    // ghosts of the initial guess under its own BCs (Dirichlet walls included), then the second
+    // buffer of the pair inherits them; wall ghosts are never rewritten afterwards
+    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), m->stream);
     HIPCHK(hipMemcpyAsync(V.t, V.u, sizeof(double) * sz, hipMemcpyDeviceToDevice, m->stream));
     launch_outer_residual(V.L, V.u, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
                           pressure_bc(m, 0, 0), m->part, m->dres, m->stream);
