@@ -31,6 +31,29 @@ __device__ inline unsigned xcd_block() {
     return b;
 }
 
+// Launch geometry over whole planes: logical block b covers slots [256 (b % bpp), +256) of plane
+// ip0 + b / bpp.  The plane index is wave-uniform (scalar division); the split of the in-plane
+// slot index uses a float reciprocal with one correction step, exact for r < 2^24.
+struct Geo {
+    unsigned bpp, per_plane, ip0, nplanes, total_blocks;
+    unsigned d1;
+    float inv1;
+};
+
+__device__ inline unsigned fdivu(unsigned r, unsigned d, float inv) {
+    unsigned q = (unsigned)((float)r * inv);
+    int rem = (int)r - (int)(q * d);
+    q += (unsigned)(rem >= (int)d) - (unsigned)(rem < 0);
+    return q;
+}
+
+__device__ inline bool geo_slot(const Geo &g, unsigned b, unsigned &ip, unsigned &r) {
+    unsigned p = b / g.bpp;
+    r = (b - p * g.bpp) * 256u + threadIdx.x;
+    ip = g.ip0 + p;
+    return p < g.nplanes && r < g.per_plane;
+}
+
 // ------------------------------------------------------------------------------------------------
 // ghost composition (imposeBC order x, y, z; z outermost)
 // ------------------------------------------------------------------------------------------------
@@ -132,17 +155,15 @@ __device__ inline void write_images(const Lay &L, T *u, int ip, int jp, int kp, 
 // ZERO_START: every opposite-colour value is known to be zero (first red half sweep after the level
 // was cleared), so the update reduces to -f / sum lc and no u is read.
 template <typename T, bool ZERO_START>
-__global__ void __launch_bounds__(256) k_rb_sweep(Lay L, T *__restrict__ u, const T *__restrict__ f,
+__global__ void __launch_bounds__(256) k_rb_sweep(Lay L, Geo G, T *__restrict__ u, const T *__restrict__ f,
                                                   const T *__restrict__ cx, const T *__restrict__ cy,
                                                   const T *__restrict__ cz, int color, Bc bc) {
-    unsigned t = xcd_block() * 256u + threadIdx.x;
-    unsigned per_plane = (unsigned)L.ny * (unsigned)L.kh;
-    unsigned ipl = t / per_plane;
-    unsigned rem = t - ipl * per_plane;
-    int ip = 1 + (int)ipl;
-    if (ip > L.nx) return;
-    int jp = 1 + (int)(rem / (unsigned)L.kh);
-    int h = (int)(rem % (unsigned)L.kh);
+    unsigned ipu, r;
+    if (!geo_slot(G, xcd_block(), ipu, r)) return;
+    int ip = (int)ipu;
+    unsigned jr = fdivu(r, G.d1, G.inv1);
+    int jp = 1 + (int)jr;
+    int h = (int)(r - jr * G.d1);
     int kb = (color + 1 + ip + jp) & 1;
     int kp = 2 * h + kb;
     if (kp < 1 || kp > L.nz) return;
@@ -183,20 +204,17 @@ __device__ inline T apply_lap(const Lay &L, const T *__restrict__ u, long row_c,
 }
 
 template <typename T, typename TO>
-__global__ void __launch_bounds__(256) k_residual(Lay L, const T *__restrict__ u, const T *__restrict__ f,
+__global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
                                                   TO *__restrict__ t_out, const T *__restrict__ cx,
                                                   const T *__restrict__ cy, const T *__restrict__ cz) {
-    unsigned t = xcd_block() * 256u + threadIdx.x;
-    unsigned per_plane = (unsigned)L.ny * 2u * (unsigned)L.kh;
-    unsigned ipl = t / per_plane;
-    unsigned rem = t - ipl * per_plane;
-    int ip = 1 + (int)ipl;
-    if (ip > L.nx) return;
-    unsigned jr = rem / (2u * (unsigned)L.kh);
-    unsigned r2 = rem - jr * 2u * (unsigned)L.kh;
+    unsigned ipu, rr;
+    if (!geo_slot(G, xcd_block(), ipu, rr)) return;
+    int ip = (int)ipu;
+    unsigned jr = fdivu(rr, G.d1, G.inv1);
+    unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = (int)(r2 / (unsigned)L.kh);
-    int h = (int)(r2 % (unsigned)L.kh);
+    int c = r2 >= (unsigned)L.kh ? 1 : 0;
+    int h = (int)r2 - c * L.kh;
     int kb = (c + 1 + ip + jp) & 1;
     int kp = 2 * h + kb;
     if (kp < 1 || kp > L.nz) return;
@@ -209,18 +227,15 @@ __global__ void __launch_bounds__(256) k_residual(Lay L, const T *__restrict__ u
 // restriction: coarse f(I,J,K) = full weighting of fine t around (2I, 2J, 2K), ghosts included
 // ------------------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, const T *__restrict__ t, T *__restrict__ fc) {
-    unsigned g = xcd_block() * 256u + threadIdx.x;
-    unsigned per_plane = (unsigned)Cl.ny * 2u * (unsigned)Cl.kh;
-    unsigned ipl = g / per_plane;
-    unsigned rem = g - ipl * per_plane;
-    int ip = 1 + (int)ipl;
-    if (ip > Cl.nx) return;
-    unsigned jr = rem / (2u * (unsigned)Cl.kh);
-    unsigned r2 = rem - jr * 2u * (unsigned)Cl.kh;
+__global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T *__restrict__ t, T *__restrict__ fc) {
+    unsigned ipu, rr;
+    if (!geo_slot(G, xcd_block(), ipu, rr)) return;
+    int ip = (int)ipu;
+    unsigned jr = fdivu(rr, G.d1, G.inv1);
+    unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = (int)(r2 / (unsigned)Cl.kh);
-    int h = (int)(r2 % (unsigned)Cl.kh);
+    int c = r2 >= (unsigned)Cl.kh ? 1 : 0;
+    int h = (int)r2 - c * Cl.kh;
     int kp = 2 * h + ((c + 1 + ip + jp) & 1);
     if (kp < 1 || kp > Cl.nz) return;
     // fine padded index of the coincident node: logical 2i -> padded 2(ip-1)+1
@@ -241,19 +256,16 @@ __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, const T *__rest
 // prolongation: fine u (+)= trilinear interpolation of coarse e (core nodes only)
 // ------------------------------------------------------------------------------------------------
 template <typename T, typename TF>
-__global__ void __launch_bounds__(256) k_prolong(Lay F, Lay Cl, const T *__restrict__ e, TF *__restrict__ u,
+__global__ void __launch_bounds__(256) k_prolong(Lay F, Lay Cl, Geo G, const T *__restrict__ e, TF *__restrict__ u,
                                                  int accumulate) {
-    unsigned g = xcd_block() * 256u + threadIdx.x;
-    unsigned per_plane = (unsigned)F.ny * 2u * (unsigned)F.kh;
-    unsigned ipl = g / per_plane;
-    unsigned rem = g - ipl * per_plane;
-    int ip = 1 + (int)ipl;
-    if (ip > F.nx) return;
-    unsigned jr = rem / (2u * (unsigned)F.kh);
-    unsigned r2 = rem - jr * 2u * (unsigned)F.kh;
+    unsigned ipu, rr;
+    if (!geo_slot(G, xcd_block(), ipu, rr)) return;
+    int ip = (int)ipu;
+    unsigned jr = fdivu(rr, G.d1, G.inv1);
+    unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = (int)(r2 / (unsigned)F.kh);
-    int h = (int)(r2 % (unsigned)F.kh);
+    int c = r2 >= (unsigned)F.kh ? 1 : 0;
+    int h = (int)r2 - c * F.kh;
     int kp = 2 * h + ((c + 1 + ip + jp) & 1);
     if (kp < 1 || kp > F.nz) return;
     int i = ip - 1, j = jp - 1, k = kp - 1;
@@ -296,21 +308,19 @@ __global__ void __launch_bounds__(256) k_fill(T *__restrict__ a, long n, T v) {
 
 // a(node) += v over core nodes (core = 1) or over every padded node (core = 0)
 template <typename T>
-__global__ void __launch_bounds__(256) k_add_scalar(Lay L, T *__restrict__ a, const double *__restrict__ dv,
+__global__ void __launch_bounds__(256) k_add_scalar(Lay L, Geo G, T *__restrict__ a, const double *__restrict__ dv,
                                                     double scale, int core) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long n = lay_size(L);
-    if (t >= n) return;
-    int h = (int)(t % L.khs);
-    long r = t / L.khs;
-    int c = (int)(r & 1);
-    r >>= 1;
-    int jp = (int)(r % (L.ny + 2));
-    int ip = (int)(r / (L.ny + 2));
+    unsigned ip, r;
+    if (!geo_slot(G, blockIdx.x, ip, r)) return;
+    unsigned jp = fdivu(r, G.d1, G.inv1);
+    unsigned r2 = r - jp * G.d1;
+    int c = r2 >= (unsigned)L.khs ? 1 : 0;
+    int h = (int)r2 - c * L.khs;
     if (h >= L.kh) return;
-    int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+    int kp = 2 * h + ((c + 1 + (int)ip + (int)jp) & 1);
     if (kp > L.nz + 1) return;
-    if (core && (ip < 1 || ip > L.nx || jp < 1 || jp > L.ny || kp < 1 || kp > L.nz)) return;
+    if (core && (ip < 1 || (int)ip > L.nx || jp < 1 || (int)jp > L.ny || kp < 1 || kp > L.nz)) return;
+    long t = (long)ip * G.per_plane + r;
     a[t] = (T)((double)a[t] + scale * dv[0]);
 }
 
@@ -388,32 +398,31 @@ __device__ inline void block_reduce4(double s0, double s1, double m2, double m3,
 // level-0 residual norms: q0 = sum |Lu - f|^2, q1 = sum f^2 (core), q2 = max |Lu - f|,
 // q3 = max |f| over every padded node (computeError(1) denominator, poisson_d3.cc:437)
 template <typename T>
-__global__ void __launch_bounds__(256) k_norm_partial(Lay L, const T *__restrict__ u, const T *__restrict__ f,
+__global__ void __launch_bounds__(256) k_norm_partial(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
                                                       const T *__restrict__ cx, const T *__restrict__ cy,
                                                       const T *__restrict__ cz, double *__restrict__ part) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long n = lay_size(L);
     double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
-    if (t < n) {
-        int h = (int)(t % L.khs);
-        long r = t / L.khs;
-        int c = (int)(r & 1);
-        r >>= 1;
-        int jp = (int)(r % (L.ny + 2));
-        int ip = (int)(r / (L.ny + 2));
+    for (unsigned lb = blockIdx.x; lb < G.total_blocks; lb += gridDim.x) {
+        unsigned ipu, r;
+        if (!geo_slot(G, lb, ipu, r)) continue;
+        unsigned jpu = fdivu(r, G.d1, G.inv1);
+        unsigned r2 = r - jpu * G.d1;
+        int c = r2 >= (unsigned)L.khs ? 1 : 0;
+        int h = (int)r2 - c * L.khs;
+        int ip = (int)ipu, jp = (int)jpu;
         int kb = (c + 1 + ip + jp) & 1;
         int kp = 2 * h + kb;
-        if (h < L.kh && kp <= L.nz + 1) {
-            double fv = (double)f[t];
-            m3 = fabs(fv);
-            if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
-                long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
-                double lu = (double)apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
-                double v = fabs(lu - fv);
-                s0 = v * v;
-                s1 = fv * fv;
-                m2 = v;
-            }
+        if (h >= L.kh || kp > L.nz + 1) continue;
+        long t = (long)ip * G.per_plane + r;
+        double fv = (double)f[t];
+        m3 = fmax(m3, fabs(fv));
+        if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
+            long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+            double lu = (double)apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+            double v = fabs(lu - fv);
+            s0 += v * v;
+            s1 += fv * fv;
+            m2 = fmax(m2, v);
         }
     }
     block_reduce4(s0, s1, m2, m3, part);
@@ -422,26 +431,24 @@ __global__ void __launch_bounds__(256) k_norm_partial(Lay L, const T *__restrict
 // weighted sums over the reference right-hand side: q0 = sum w_i w_j w_k f (core), q1 = sum f over
 // every padded node
 template <typename T>
-__global__ void __launch_bounds__(256) k_rhs_sums(Lay L, const T *__restrict__ f, const double *__restrict__ wx,
+__global__ void __launch_bounds__(256) k_rhs_sums(Lay L, Geo G, const T *__restrict__ f, const double *__restrict__ wx,
                                                   const double *__restrict__ wy, const double *__restrict__ wz,
                                                   double *__restrict__ part) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long n = lay_size(L);
     double s0 = 0, s1 = 0;
-    if (t < n) {
-        int h = (int)(t % L.khs);
-        long r = t / L.khs;
-        int c = (int)(r & 1);
-        r >>= 1;
-        int jp = (int)(r % (L.ny + 2));
-        int ip = (int)(r / (L.ny + 2));
+    for (unsigned lb = blockIdx.x; lb < G.total_blocks; lb += gridDim.x) {
+        unsigned ipu, r;
+        if (!geo_slot(G, lb, ipu, r)) continue;
+        unsigned jpu = fdivu(r, G.d1, G.inv1);
+        unsigned r2 = r - jpu * G.d1;
+        int c = r2 >= (unsigned)L.khs ? 1 : 0;
+        int h = (int)r2 - c * L.khs;
+        int ip = (int)ipu, jp = (int)jpu;
         int kp = 2 * h + ((c + 1 + ip + jp) & 1);
-        if (h < L.kh && kp <= L.nz + 1) {
-            double fv = (double)f[t];
-            s1 = fv;
-            if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz && wx)
-                s0 = wx[ip - 1] * wy[jp - 1] * wz[kp - 1] * fv;
-        }
+        if (h >= L.kh || kp > L.nz + 1) continue;
+        double fv = (double)f[(long)ip * G.per_plane + r];
+        s1 += fv;
+        if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz && wx)
+            s0 += wx[ip - 1] * wy[jp - 1] * wz[kp - 1] * fv;
     }
     block_reduce4(s0, s1, 0.0, 0.0, part);
 }
@@ -549,20 +556,17 @@ __global__ void __launch_bounds__(256) k_coarsest(Lay L, T *__restrict__ ug, con
 // Jacobi sweep t = J(u) on core nodes (poisson_d3.cc:152-166)
 // ------------------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_jacobi(Lay L, const T *__restrict__ u, const T *__restrict__ f,
+__global__ void __launch_bounds__(256) k_jacobi(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
                                                 T *__restrict__ tout, const T *__restrict__ cx,
                                                 const T *__restrict__ cy, const T *__restrict__ cz) {
-    unsigned g = xcd_block() * 256u + threadIdx.x;
-    unsigned per_plane = (unsigned)L.ny * 2u * (unsigned)L.kh;
-    unsigned ipl = g / per_plane;
-    unsigned rem = g - ipl * per_plane;
-    int ip = 1 + (int)ipl;
-    if (ip > L.nx) return;
-    unsigned jr = rem / (2u * (unsigned)L.kh);
-    unsigned r2 = rem - jr * 2u * (unsigned)L.kh;
+    unsigned ipu, rr;
+    if (!geo_slot(G, xcd_block(), ipu, rr)) return;
+    int ip = (int)ipu;
+    unsigned jr = fdivu(rr, G.d1, G.inv1);
+    unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = (int)(r2 / (unsigned)L.kh);
-    int h = (int)(r2 % (unsigned)L.kh);
+    int c = r2 >= (unsigned)L.kh ? 1 : 0;
+    int h = (int)r2 - c * L.kh;
     int kb = (c + 1 + ip + jp) & 1;
     int kp = 2 * h + kb;
     if (kp < 1 || kp > L.nz) return;
@@ -609,46 +613,45 @@ __global__ void __launch_bounds__(256) k_project_small(Lay L, T *__restrict__ f,
 // sum r^2, sum f^2 (core), max |r| (core), max |f| (every padded node) in FP64.
 // ------------------------------------------------------------------------------------------------
 template <bool CORR>
-__global__ void __launch_bounds__(256) k_outer_residual(Lay L, const double *__restrict__ u_in,
+__global__ void __launch_bounds__(256) k_outer_residual(Lay L, Geo G, const double *__restrict__ u_in,
                                                         const float *__restrict__ e, const double *__restrict__ f,
                                                         double *__restrict__ u_out, float *__restrict__ r32,
                                                         const double *__restrict__ cx, const double *__restrict__ cy,
                                                         const double *__restrict__ cz, Bc bc,
                                                         double *__restrict__ part) {
-    long t = (long)xcd_block() * 256 + threadIdx.x;
-    long n = lay_size(L);
     double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
-    if (t < n) {
-        int h = (int)(t % L.khs);
-        long rr = t / L.khs;
-        int c = (int)(rr & 1);
-        rr >>= 1;
-        int jp = (int)(rr % (L.ny + 2));
-        int ip = (int)(rr / (L.ny + 2));
+    const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
+    for (unsigned lb = blockIdx.x; lb < G.total_blocks; lb += gridDim.x) {
+        unsigned ipu, r;
+        if (!geo_slot(G, lb, ipu, r)) continue;
+        unsigned jpu = fdivu(r, G.d1, G.inv1);
+        unsigned r2 = r - jpu * G.d1;
+        int c = r2 >= (unsigned)L.khs ? 1 : 0;
+        int h = (int)r2 - c * L.khs;
+        int ip = (int)ipu, jp = (int)jpu;
         int kb = (c + 1 + ip + jp) & 1;
         int kp = 2 * h + kb;
-        if (h < L.kh && kp <= L.nz + 1) {
-            double fv = f[t];
-            m3 = fabs(fv);
-            if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
-                long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
-                long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
-                auto U = [&](long q) { return CORR ? u_in[q] + (double)e[q] : u_in[q]; };
-                double uc = U(rc + h);
-                const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
-                double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) +
-                            Y[1] * U(ro - sy + h) + Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) -
-                            (X[2] + Y[2] + Z[2]) * uc;
-                double r = fv - lu;
-                r32[t] = (float)r;
-                if (CORR) {
-                    u_out[t] = uc;
-                    write_images(L, u_out, ip, jp, kp, uc, bc);
-                }
-                s0 = r * r;
-                s1 = fv * fv;
-                m2 = fabs(r);
+        if (h >= L.kh || kp > L.nz + 1) continue;
+        long t = (long)ip * G.per_plane + r;
+        double fv = f[t];
+        m3 = fmax(m3, fabs(fv));
+        if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
+            long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+            auto U = [&](long q) { return CORR ? u_in[q] + (double)e[q] : u_in[q]; };
+            double uc = U(rc + h);
+            const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+            double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) +
+                        Y[1] * U(ro - sy + h) + Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) -
+                        (X[2] + Y[2] + Z[2]) * uc;
+            double rv = fv - lu;
+            r32[t] = (float)rv;
+            if (CORR) {
+                u_out[t] = uc;
+                write_images(L, u_out, ip, jp, kp, uc, bc);
             }
+            s0 += rv * rv;
+            s1 += fv * fv;
+            m2 = fmax(m2, fabs(rv));
         }
     }
     block_reduce4(s0, s1, m2, m3, part);
@@ -659,6 +662,26 @@ __global__ void __launch_bounds__(256) k_outer_residual(Lay L, const double *__r
 // ------------------------------------------------------------------------------------------------
 static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+static Geo geo_planes(unsigned per_plane, unsigned ip0, unsigned nplanes, unsigned d1) {
+    Geo g;
+    g.per_plane = per_plane;
+    g.bpp = (per_plane + 255u) / 256u;
+    g.ip0 = ip0;
+    g.nplanes = nplanes;
+    g.total_blocks = g.bpp * nplanes;
+    g.d1 = d1;
+    g.inv1 = 1.0f / (float)d1;
+    return g;
+}
+// one colour of the core nodes: planes 1..nx, slots (jp, h) with jp in 1..ny
+static Geo geo_sweep(const Lay &L) { return geo_planes((unsigned)(L.ny * L.kh), 1, L.nx, L.kh); }
+// both colours of the core nodes: slots (jp, c, h)
+static Geo geo_core(const Lay &L) { return geo_planes((unsigned)(L.ny * 2 * L.kh), 1, L.nx, 2 * L.kh); }
+// every stored slot of the padded level
+static Geo geo_all(const Lay &L) { return geo_planes((unsigned)((L.ny + 2) * 2 * L.khs), 0, L.nx + 2, 2 * L.khs); }
+static const unsigned REDUCE_BLOCKS = 2048;
+static unsigned reduce_grid(const Geo &g) { return g.total_blocks < REDUCE_BLOCKS ? g.total_blocks : REDUCE_BLOCKS; }
+
 template <typename T>
 void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s) {
     long total = 2L * (L.nx + 2) * (L.ny + 2) + 2L * (L.nx + 2) * L.nz + 2L * L.ny * L.nz;
@@ -668,30 +691,30 @@ void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s) {
 template <typename T>
 void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
                      const Bc &bc, bool zero_start, hipStream_t s) {
-    long total = (long)L.nx * L.ny * L.kh;
+    Geo G = geo_sweep(L);
     if (zero_start)
-        hipLaunchKernelGGL((k_rb_sweep<T, true>), dim3(nblk(total)), dim3(256), 0, s, L, u, f, cx, cy, cz, color, bc);
+        hipLaunchKernelGGL((k_rb_sweep<T, true>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, cx, cy, cz, color, bc);
     else
-        hipLaunchKernelGGL((k_rb_sweep<T, false>), dim3(nblk(total)), dim3(256), 0, s, L, u, f, cx, cy, cz, color, bc);
+        hipLaunchKernelGGL((k_rb_sweep<T, false>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, cx, cy, cz, color, bc);
 }
 
 template <typename T, typename TO>
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
                      hipStream_t s) {
-    long total = (long)L.nx * L.ny * 2 * L.kh;
-    hipLaunchKernelGGL((k_residual<T, TO>), dim3(nblk(total)), dim3(256), 0, s, L, u, f, t, cx, cy, cz);
+    Geo G = geo_core(L);
+    hipLaunchKernelGGL((k_residual<T, TO>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz);
 }
 
 template <typename T>
 void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s) {
-    long total = (long)Cl.nx * Cl.ny * 2 * Cl.kh;
-    hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(total)), dim3(256), 0, s, F, Cl, t, fc);
+    Geo G = geo_core(Cl);
+    hipLaunchKernelGGL(k_restrict<T>, dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc);
 }
 
 template <typename T, typename TF>
 void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s) {
-    long total = (long)F.nx * F.ny * 2 * F.kh;
-    hipLaunchKernelGGL((k_prolong<T, TF>), dim3(nblk(total)), dim3(256), 0, s, F, Cl, e, u, accumulate);
+    Geo G = geo_core(F);
+    hipLaunchKernelGGL((k_prolong<T, TF>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, e, u, accumulate);
 }
 
 template <typename T>
@@ -704,7 +727,8 @@ void launch_fill(T *a, long n, T v, hipStream_t s) {
 
 template <typename T>
 void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int core, hipStream_t s) {
-    hipLaunchKernelGGL(k_add_scalar<T>, dim3(nblk(lay_size(L))), dim3(256), 0, s, L, a, dv, scale, core);
+    Geo G = geo_all(L);
+    hipLaunchKernelGGL(k_add_scalar<T>, dim3(G.total_blocks), dim3(256), 0, s, L, G, a, dv, scale, core);
 }
 
 template <typename TS, typename TD>
@@ -719,21 +743,23 @@ void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s) {
     hipLaunchKernelGGL((k_unpack<TS, TD>), dim3(nblk(n)), dim3(256), 0, s, L, src, ref);
 }
 
-int norm_parts(const Lay &L) { return (int)nblk(lay_size(L)); }
+int norm_parts(const Lay &L) { return (int)reduce_grid(geo_all(L)); }
 
 template <typename T>
 void launch_norm(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part,
                  double *out, hipStream_t s) {
-    int nb = norm_parts(L);
-    hipLaunchKernelGGL(k_norm_partial<T>, dim3(nb), dim3(256), 0, s, L, u, f, cx, cy, cz, part);
+    Geo G = geo_all(L);
+    int nb = (int)reduce_grid(G);
+    hipLaunchKernelGGL(k_norm_partial<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 
 template <typename T>
 void launch_rhs_sums(const Lay &L, const T *f, const double *wx, const double *wy, const double *wz, double *part,
                      double *out, hipStream_t s) {
-    int nb = norm_parts(L);
-    hipLaunchKernelGGL(k_rhs_sums<T>, dim3(nb), dim3(256), 0, s, L, f, wx, wy, wz, part);
+    Geo G = geo_all(L);
+    int nb = (int)reduce_grid(G);
+    hipLaunchKernelGGL(k_rhs_sums<T>, dim3(nb), dim3(256), 0, s, L, G, f, wx, wy, wz, part);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 
@@ -752,8 +778,8 @@ void launch_coarsest(const Lay &L, T *u, const T *f, const T *cx, const T *cy, c
 template <typename T>
 void launch_jacobi(const Lay &L, const T *u, const T *f, T *t, const T *cx, const T *cy, const T *cz,
                    hipStream_t s) {
-    long total = (long)L.nx * L.ny * 2 * L.kh;
-    hipLaunchKernelGGL(k_jacobi<T>, dim3(nblk(total)), dim3(256), 0, s, L, u, f, t, cx, cy, cz);
+    Geo G = geo_core(L);
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz);
 }
 
 template <typename T>
@@ -765,12 +791,13 @@ void launch_project_small(const Lay &L, T *f, const double *wx, const double *wy
 void launch_outer_residual(const Lay &L, const double *u_in, const float *e, const double *f, double *u_out,
                            float *r32, const double *cx, const double *cy, const double *cz, const Bc &bc,
                            double *part, double *out, hipStream_t s) {
-    int nb = norm_parts(L);
+    Geo G = geo_all(L);
+    int nb = (int)reduce_grid(G);
     if (e)
-        hipLaunchKernelGGL(k_outer_residual<true>, dim3(nb), dim3(256), 0, s, L, u_in, e, f, u_out, r32, cx, cy, cz,
-                           bc, part);
+        hipLaunchKernelGGL(k_outer_residual<true>, dim3(nb), dim3(256), 0, s, L, G, u_in, e, f, u_out, r32, cx, cy,
+                           cz, bc, part);
     else
-        hipLaunchKernelGGL(k_outer_residual<false>, dim3(nb), dim3(256), 0, s, L, u_in, e, f, u_out, r32, cx, cy,
+        hipLaunchKernelGGL(k_outer_residual<false>, dim3(nb), dim3(256), 0, s, L, G, u_in, e, f, u_out, r32, cx, cy,
                            cz, bc, part);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
