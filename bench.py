@@ -94,6 +94,8 @@ def main():
     bytes_sweep = solver.kernel_bytes(0, 0)
     achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else None
     iso_ms = solver.time_kernel(0, 0, 20)
+    plan = {"coarse_cycle_first_level_fp64": solver.info(0), "coarse_cycle_lds_bytes_fp64": solver.info(2),
+            "lds_limit_per_workgroup": solver.info(4)}
     solver.close()
     # whole-cycle algorithmic traffic (130 B per node per level + 16 B per finest node for the norm)
     n_levels = [((1 << (8 - l)) + 1) ** 3 for l in range(p.vcDepth + 1)]
@@ -132,7 +134,7 @@ def main():
                                "V(2,2) RB-GS to 1e-10 rel. residual (plain FP64 multigrid)",
                    "nodes": [257, 257, 257], "vcDepth": p.vcDepth, "variant": "fp64-mg",
                    "cycles_per_solve": int(np.median(cycles)), "final_rel_residual": res, "converged": conv,
-                   "solve_GBs_algorithmic": round(solve_gbs, 1), "mixed_precision": mixed},
+                   "solve_GBs_algorithmic": round(solve_gbs, 1), "mixed_precision": mixed, "plan": plan},
         "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<double> finest level",
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,

@@ -123,6 +123,11 @@ int sr_mg_time_kernel(sr_mg *mg, int kernel, int level, int reps, double *avg_ms
 /* Algorithmic bytes one launch of that kernel moves (the roofline numerator).                   */
 int sr_mg_kernel_bytes(const sr_mg *mg, int kernel, int level, double *bytes);
 
+/* Plan facts of a handle: what 0 = first level of the one-workgroup coarse cycle (FP64 hierarchy),
+ * 1 = same for the FP32 hierarchy (a value above vcDepth means per-level launches throughout),
+ * 2 / 3 = LDS bytes of those kernels, 4 = the device's LDS limit per workgroup.                   */
+int sr_mg_info(const sr_mg *mg, int what, long *value);
+
 /* ---- grid ------------------------------------------------------------------------------------ */
 /* One direction of lib/grid/grid.cc: faces = 0 gives the n node ("staggered") coordinates and
  * metrics (padded length n+2, index -1..n); faces = 1 the n-1 face ("collocated") ones (padded

@@ -62,6 +62,7 @@ SIGNATURES = [
     ("sr_mg_op_vcycle", C.c_int, [_vp]),
     ("sr_mg_time_kernel", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     ("sr_mg_kernel_bytes", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    ("sr_mg_info", C.c_int, [_vp, C.c_int, C.POINTER(C.c_long)]),
     ("sr_grid_axis", C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
     ("sr_device_alloc", C.c_int, [C.c_ulong, C.POINTER(C.c_void_p)]),
     ("sr_device_free", C.c_int, [C.c_void_p]),

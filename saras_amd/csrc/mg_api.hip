@@ -85,6 +85,46 @@ static int upload_vec(T **dst, const std::vector<T> &v) {
     return SR_OK;
 }
 
+// Coarse end of the V-cycle in one workgroup: the largest level index range first..depth whose u, f
+// and coefficients fit the workgroup's LDS (first >= 1: level 0 carries the solution itself).
+template <typename T>
+static void plan_coarse(sr_mg *m, int &first, CoarseDesc &D, size_t &lds, size_t limit) {
+    first = 1 << 20;
+    int depth = m->depth;
+    for (int f = 1; f < depth; f++) {
+        CoarseDesc t{};
+        t.first = f;
+        t.last = depth;
+        if (depth - f + 1 > 8) continue;
+        for (int q = 0; q <= depth - f; q++) t.L[q] = m->lv[f + q].L;
+        size_t bytes = coarse_cycle_plan<T>(t);
+        if (bytes <= limit) {
+            first = f;
+            D = t;
+            lds = bytes;
+            break;
+        }
+    }
+}
+
+static void finish_coarse(sr_mg *m, CoarseDesc &D, bool fp32) {
+    for (int q = 0; q <= D.last - D.first; q++)
+        for (int a = 0; a < 3; a++)
+            D.cg[q][a] = fp32 ? (const void *)m->lv[D.first + q].c32[a] : (const void *)m->lv[D.first + q].c64[a];
+    D.pre = m->d.preSmooth;
+    D.post = m->d.postSmooth;
+    D.bc = pressure_bc(m, D.first, 1);
+    for (int q = 0; q < 6; q++) D.bc.wall[q] = nullptr;
+    D.tbc = tmp_bc(m);
+    D.tol = m->d.mgTolerance;
+    D.max_count = m->maxCount;
+    bool sing = !(m->d.testPoisson && !(m->d.xPer && m->d.yPer && m->d.zPer));
+    D.project = (m->d.coarseProject && sing) ? 1 : 0;
+    Level &Cv = m->lv[m->depth];
+    for (int a = 0; a < 3; a++) D.wc[a] = Cv.wc[a];
+    D.wcsum = Cv.wcsum;
+}
+
 static int create_impl(const sr_mg_desc *d, sr_mg *m) {
     m->d = *d;
     for (int q = 0; q < 6; q++) m->d.wall[q] = nullptr;
@@ -155,6 +195,27 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
         }
     }
     if (d->precision == SR_PREC_MIXED) RETCHK(ensure_fp32(m));
+    // coarse end of the cycle in LDS
+    {
+        int dev = 0, lds_max = 65536;
+        HIPCHK(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds_max = 65536;
+        m->lds_limit = lds_max;
+        size_t limit = (size_t)lds_max - 1024;   // static LDS of the kernel
+        if (d->gsSmooth) {
+            plan_coarse<double>(m, m->coarse_first64, m->cdesc64, m->clds64, limit);
+            if (m->coarse_first64 <= depth) {
+                finish_coarse(m, m->cdesc64, false);
+                if (coarse_cycle_attr<double>(m->clds64)) m->coarse_first64 = 1 << 20;
+            }
+            plan_coarse<float>(m, m->coarse_first32, m->cdesc32, m->clds32, limit);
+            if (m->coarse_first32 <= depth) {
+                finish_coarse(m, m->cdesc32, true);
+                if (coarse_cycle_attr<float>(m->clds32)) m->coarse_first32 = 1 << 20;
+            }
+        }
+        (void)hipGetLastError();
+    }
     // null vectors for the range projection (finest level)
     const Level &V0 = m->lv[0];
     int per[3] = {d->xPer, d->yPer, d->zPer};
@@ -351,11 +412,15 @@ static void project_coarsest(sr_mg *m) {
 void sr::vcycle_fp64(sr_mg *m) {
     const sr_mg_desc &d = m->d;
     int D = m->depth;
+    int Lc = (m->coarse_enable && d.gsSmooth) ? m->coarse_first64 : (1 << 20);
+    int bottom = Lc <= D ? Lc : D;   // last level handled by per-level launches + 1
     smooth_level(m, 0, d.preSmooth, 0, false, false, m->sw0, m->sw1);
-    for (int l = 0; l < D; l++) {
+    for (int l = 0; l < bottom; l++) {
         residual_level(m, l);
         launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream);
-        if (l + 1 == D) {
+        if (l + 1 == Lc) {
+            launch_coarse_cycle<double>(m->cdesc64, m->clds64, m->lv[Lc].u, m->lv[Lc].f, m->diters, m->stream);
+        } else if (l + 1 == D) {
             launch_fill<double>(m->lv[D].u, lay_size(m->lv[D].L), 0.0, m->stream);
             if (d.coarseProject && singular(m)) project_coarsest(m);
             coarsest_level(m, D, 1);
@@ -366,7 +431,7 @@ void sr::vcycle_fp64(sr_mg *m) {
             smooth_level(m, l + 1, d.preSmooth, 1, false, false);
         }
     }
-    for (int l = D - 1; l >= 0; l--) {
+    for (int l = bottom - 1; l >= 0; l--) {
         launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream);
         smooth_level(m, l, d.postSmooth, l == 0 ? 0 : 1, false, false);
     }
@@ -405,13 +470,17 @@ void sr::vcycle_fp32(sr_mg *m) {
     const sr_mg_desc &d = m->d;
     int D = m->depth;
     hipStream_t s = m->stream;
+    int Lc = m->coarse_enable ? m->coarse_first32 : (1 << 20);
+    int bottom = Lc <= D ? Lc : D;
     smooth32(m, 0, d.preSmooth, 0, true, m->sw0, m->sw1);
-    for (int l = 0; l < D; l++) {
+    for (int l = 0; l < bottom; l++) {
         Level &V = m->lv[l];
         launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s);
         launch_fill_ghosts<float>(V.L, V.t32, tmp_bc(m), s);
         launch_restrict<float>(V.L, m->lv[l + 1].L, V.t32, m->lv[l + 1].f32, s);
-        if (l + 1 == D) {
+        if (l + 1 == Lc) {
+            launch_coarse_cycle<float>(m->cdesc32, m->clds32, m->lv[Lc].u32, m->lv[Lc].f32, m->diters, s);
+        } else if (l + 1 == D) {
             Level &Cv = m->lv[D];
             launch_fill<float>(Cv.u32, lay_size(Cv.L), 0.0f, s);
             if (d.coarseProject && singular(m))
@@ -422,7 +491,7 @@ void sr::vcycle_fp32(sr_mg *m) {
             smooth32(m, l + 1, d.preSmooth, 1, true);
         }
     }
-    for (int l = D - 1; l >= 0; l--) {
+    for (int l = bottom - 1; l >= 0; l--) {
         launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s);
         smooth32(m, l, d.postSmooth, l == 0 ? 0 : 1, false);
     }
@@ -604,6 +673,19 @@ extern "C" int sr_mg_op_vcycle(sr_mg *m) {
     m->vLevel = 0;
     m->zeroBC = 0;
     return sync_check(m);
+}
+
+extern "C" int sr_mg_info(const sr_mg *m, int what, long *value) {
+    if (!m || !value) return set_err(SR_ERR_ARG, "null argument");
+    switch (what) {
+        case 0: *value = m->coarse_first64; break;
+        case 1: *value = m->coarse_first32; break;
+        case 2: *value = (long)m->clds64; break;
+        case 3: *value = (long)m->clds32; break;
+        case 4: *value = m->lds_limit; break;
+        default: return set_err(SR_ERR_ARG, "unknown info key");
+    }
+    return SR_OK;
 }
 
 extern "C" int sr_mg_history(const sr_mg *m, double *out, int cap) {

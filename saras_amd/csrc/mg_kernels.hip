@@ -658,6 +658,251 @@ __global__ void __launch_bounds__(256) k_outer_residual(Lay L, Geo G, const doub
 }
 
 // ------------------------------------------------------------------------------------------------
+// the coarse end of the V-cycle in one workgroup: levels first..last of the down leg, the coarsest
+// solve and the up leg back to `first`, with every level's u and f held in LDS
+// (poisson.cc:239-279 restricted to those levels; poisson_d3.cc smooth/coarsen/solve/prolong)
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline void blk_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
+                                 const Bc &bc, bool zero_start) {
+    int n = L.nx * L.ny * L.kh;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int h = t % L.kh, r = t / L.kh, jp = 1 + r % L.ny, ip = 1 + r / L.ny;
+        int kb = (color + 1 + ip + jp) & 1;
+        int kp = 2 * h + kb;
+        if (kp < 1 || kp > L.nz) continue;
+        long rc = lay_row(L, ip, jp, color);
+        const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        T num;
+        if (zero_start) {
+            num = -f[rc + h];
+        } else {
+            long ro = lay_row(L, ip, jp, color ^ 1);
+            long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
+            num = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
+                  Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - f[rc + h];
+        }
+        T v = num / (X[2] + Y[2] + Z[2]);
+        u[rc + h] = v;
+        write_images(L, u, ip, jp, kp, v, bc);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline void blk_zero(const Lay &L, T *u) {
+    long n = lay_size(L);
+    for (long t = threadIdx.x; t < n; t += blockDim.x) u[t] = T(0);
+    __syncthreads();
+}
+
+// residual of level L at padded node (ip, jp, kp) as the restriction reads it: ghost positions
+// follow the residual-pad rules (zero, or the value at the periodic / mirror source)
+template <typename T>
+__device__ inline T blk_res_at(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz,
+                               const Bc &tb, int ip, int jp, int kp) {
+    int n[3] = {L.nx, L.ny, L.nz};
+    int q[3] = {ip, jp, kp};
+    T sign = T(1);
+    for (int d = 2; d >= 0; --d) {
+        int g = q[d];
+        if (g != 0 && g != n[d] + 1) continue;
+        int r = tb.rule[d];
+        if (r == G_ZERO || r == G_WALL || r == G_KEEP) return T(0);
+        q[d] = ghost_src(r, g, n[d]);
+        if (r == G_ANTI) sign = -sign;
+    }
+    int c = lay_color(q[0], q[1], q[2]);
+    long rc = lay_row(L, q[0], q[1], c), ro = lay_row(L, q[0], q[1], c ^ 1);
+    int h = q[2] >> 1, kb = q[2] & 1;
+    T lu = apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * q[0], cy + 4 * q[1], cz + 4 * q[2]);
+    return sign * (f[rc + h] - lu);
+}
+
+template <typename T>
+__device__ inline void blk_restrict(const Lay &F, const T *uf, const T *ff, const T *cx, const T *cy, const T *cz,
+                                    const Bc &tb, const Lay &Cl, T *fc) {
+    int n = Cl.nx * Cl.ny * Cl.nz;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % Cl.nz, r = t / Cl.nz, j = r % Cl.ny, i = r / Cl.ny;
+        int fi = 2 * i + 1, fj = 2 * j + 1, fk = 2 * k + 1;
+#define RR(a, b, e) blk_res_at<T>(F, uf, ff, cx, cy, cz, tb, fi + (a), fj + (b), fk + (e))
+        T facePoints = (RR(1, 0, 0) + RR(-1, 0, 0) + RR(0, 1, 0) + RR(0, -1, 0) + RR(0, 0, 1) + RR(0, 0, -1)) * T(0.0625);
+        T edgePoints = (RR(1, 1, 0) + RR(1, -1, 0) + RR(-1, -1, 0) + RR(-1, 1, 0) + RR(0, 1, 1) + RR(0, -1, 1) +
+                        RR(0, -1, -1) + RR(0, 1, -1) + RR(1, 0, 1) + RR(1, 0, -1) + RR(-1, 0, -1) + RR(-1, 0, 1)) *
+                       T(0.03125);
+        T vertPoints = (RR(1, 1, 1) + RR(1, 1, -1) + RR(1, -1, 1) + RR(-1, 1, 1) + RR(1, -1, -1) + RR(-1, 1, -1) +
+                        RR(-1, -1, 1) + RR(-1, -1, -1)) * T(0.015625);
+        fc[lay_idx(Cl, i + 1, j + 1, k + 1)] = facePoints + edgePoints + vertPoints + RR(0, 0, 0) * T(0.125);
+#undef RR
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const T *e) {
+    int n = F.nx * F.ny * F.nz;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % F.nz, r = t / F.nz, j = r % F.ny, i = r / F.ny;
+        int io = i & 1, jo = j & 1, ko = k & 1;
+        int ci = i / 2 + 1, cj = j / 2 + 1, ck = k / 2 + 1;
+#define EE(a, b, q) e[lay_idx(Cl, ci + (a), cj + (b), ck + (q))]
+        T v;
+        if (io && jo && ko)
+            v = (EE(0, 0, 0) + EE(1, 0, 0) + EE(0, 1, 0) + EE(0, 0, 1) + EE(1, 1, 0) + EE(1, 0, 1) + EE(0, 1, 1) +
+                 EE(1, 1, 1)) / T(8.0);
+        else if (io && jo)
+            v = (EE(0, 0, 0) + EE(0, 1, 0) + EE(1, 0, 0) + EE(1, 1, 0)) / T(4.0);
+        else if (io && ko)
+            v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(1, 0, 0) + EE(1, 0, 1)) / T(4.0);
+        else if (io)
+            v = (EE(0, 0, 0) + EE(1, 0, 0)) / T(2.0);
+        else if (jo && ko)
+            v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(0, 1, 0) + EE(0, 1, 1)) / T(4.0);
+        else if (jo)
+            v = (EE(0, 0, 0) + EE(0, 1, 0)) / T(2.0);
+        else if (ko)
+            v = (EE(0, 0, 0) + EE(0, 0, 1)) / T(2.0);
+        else
+            v = EE(0, 0, 0);
+#undef EE
+        long o = lay_idx(F, i + 1, j + 1, k + 1);
+        uf[o] = uf[o] + v;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline double blk_max_res(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz,
+                                     double *red) {
+    double m = -1.0e-10;
+    int nc = L.nx * L.ny * L.nz;
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int ip = i + 1, jp = j + 1, kp = k + 1;
+        int c = lay_color(ip, jp, kp);
+        long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+        double lu = (double)apply_lap<T, T>(L, u, rc, ro, kp >> 1, kp & 1, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+        m = fmax(m, fabs((double)f[rc + (kp >> 1)] - lu));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double gm = red[0];
+    for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) gm = fmax(gm, red[w]);
+    __syncthreads();
+    return gm;
+}
+
+template <typename T>
+__device__ inline void blk_project(const Lay &L, T *f, const double *wx, const double *wy, const double *wz,
+                                   double wsum, double *red) {
+    int n = L.nx * L.ny * L.nz;
+    double s = 0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        s += wx[i] * wy[j] * wz[k] * (double)f[lay_idx(L, i + 1, j + 1, k + 1)];
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double tot = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) tot += red[w];
+    __syncthreads();
+    double c = tot / wsum;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        long o = lay_idx(L, i + 1, j + 1, k + 1);
+        f[o] = (T)((double)f[o] - c);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restrict__ u_first, const T *__restrict__ f_first,
+                                                       int *__restrict__ iters) {
+    extern __shared__ unsigned char smem[];
+    T *base = (T *)smem;
+    __shared__ double red[16];
+    const int nl = D.last - D.first + 1;
+    // coefficients and the right-hand side of the first level into LDS; u(first) starts at zero
+    for (int q = 0; q < nl; q++) {
+        const Lay &L = D.L[q];
+        int nn[3] = {L.nx + 2, L.ny + 2, L.nz + 2};
+        for (int a = 0; a < 3; a++) {
+            const T *src = (const T *)D.cg[q][a];
+            T *dst = base + D.off_c[q][a];
+            for (int t = threadIdx.x; t < 4 * nn[a]; t += blockDim.x) dst[t] = src[t];
+        }
+    }
+    {
+        long n = lay_size(D.L[0]);
+        T *f = base + D.off_f[0];
+        for (long t = threadIdx.x; t < n; t += blockDim.x) f[t] = f_first[t];
+    }
+    __syncthreads();
+#define CX(q) (base + D.off_c[q][0])
+#define CY(q) (base + D.off_c[q][1])
+#define CZ(q) (base + D.off_c[q][2])
+#define UU(q) (base + D.off_u[q])
+#define FF(q) (base + D.off_f[q])
+    // down leg
+    for (int q = 0; q + 1 < nl; q++) {
+        const Lay &L = D.L[q];
+        for (int s = 0; s < D.pre; s++) {
+            blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0, D.bc, s == 0);
+            blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 1, D.bc, false);
+        }
+        if (D.pre == 0) {
+            blk_zero<T>(L, UU(q));
+        }
+        blk_restrict<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, D.L[q + 1], FF(q + 1));
+    }
+    // coarsest solve (poisson_d3.cc:190-247 with the red-black sweep)
+    {
+        int q = nl - 1;
+        const Lay &L = D.L[q];
+        blk_zero<T>(L, UU(q));
+        if (D.project) blk_project<T>(L, FF(q), D.wc[0], D.wc[1], D.wc[2], D.wcsum, red);
+        int it = 0;
+        while (true) {
+            lds_fill_ghosts(L, UU(q), D.bc);
+            lds_sweep(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0);
+            __syncthreads();
+            lds_fill_ghosts(L, UU(q), D.bc);
+            lds_sweep(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 1);
+            __syncthreads();
+            double gm = blk_max_res<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), red);
+            if (gm < D.tol) break;
+            it += 1;
+            if (it > D.max_count) break;
+        }
+        lds_fill_ghosts(L, UU(q), D.bc);
+        if (threadIdx.x == 0) iters[0] = it;
+    }
+    // up leg
+    for (int q = nl - 2; q >= 0; q--) {
+        const Lay &L = D.L[q];
+        blk_prolong_add<T>(L, UU(q), D.L[q + 1], UU(q + 1));
+        lds_fill_ghosts(L, UU(q), D.bc);
+        for (int s = 0; s < D.post; s++) {
+            blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0, D.bc, false);
+            blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 1, D.bc, false);
+        }
+    }
+    {
+        long n = lay_size(D.L[0]);
+        T *u = UU(0);
+        for (long t = threadIdx.x; t < n; t += blockDim.x) u_first[t] = u[t];
+    }
+#undef CX
+#undef CY
+#undef CZ
+#undef UU
+#undef FF
+}
+
+// ------------------------------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------------------------------
 static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -802,6 +1047,32 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 
+template <typename T>
+size_t coarse_cycle_plan(CoarseDesc &D) {
+    size_t off = 0;
+    auto take = [&](size_t n) { size_t o = off; off += (n + 7) & ~(size_t)7; return (int)o; };
+    for (int q = 0; q <= D.last - D.first; q++) {
+        const Lay &L = D.L[q];
+        D.off_u[q] = take((size_t)lay_size(L));
+        D.off_f[q] = take((size_t)lay_size(L));
+        D.off_c[q][0] = take(4 * (size_t)(L.nx + 2));
+        D.off_c[q][1] = take(4 * (size_t)(L.ny + 2));
+        D.off_c[q][2] = take(4 * (size_t)(L.nz + 2));
+    }
+    return off * sizeof(T);
+}
+
+template <typename T>
+void launch_coarse_cycle(const CoarseDesc &D, size_t lds, T *u_first, const T *f_first, int *iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(1), dim3(1024), lds, s, D, u_first, f_first, iters);
+}
+
+template <typename T>
+int coarse_cycle_attr(size_t lds) {
+    return hipFuncSetAttribute((const void *)k_coarse_cycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+                   hipSuccess ? 0 : 1;
+}
+
 // explicit instantiations
 #define SR_INST(T)                                                                                          \
     template void launch_fill_ghosts<T>(const Lay &, T *, const Bc &, hipStream_t);                        \
@@ -823,6 +1094,12 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
                                           double, hipStream_t);
 SR_INST(double)
 SR_INST(float)
+template size_t coarse_cycle_plan<double>(CoarseDesc &);
+template size_t coarse_cycle_plan<float>(CoarseDesc &);
+template void launch_coarse_cycle<double>(const CoarseDesc &, size_t, double *, const double *, int *, hipStream_t);
+template void launch_coarse_cycle<float>(const CoarseDesc &, size_t, float *, const float *, int *, hipStream_t);
+template int coarse_cycle_attr<double>(size_t);
+template int coarse_cycle_attr<float>(size_t);
 #undef SR_INST
 
 template void launch_residual<double, double>(const Lay &, const double *, const double *, double *,

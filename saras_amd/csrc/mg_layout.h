@@ -57,4 +57,21 @@ __host__ __device__ inline int ghost_src(int rule, int g, int n) {
     return g == 0 ? 2 : n - 1;   // mirror / anti
 }
 
+// Levels first..last (at most 8) of the coarse end of a V-cycle run by one workgroup in LDS.
+// Offsets are in elements of the level type.
+struct CoarseDesc {
+    int first, last;
+    Lay L[8];
+    int off_u[8], off_f[8], off_c[8][3];
+    const void *cg[8][3];
+    int pre, post;
+    Bc bc;       // smoothing ghosts of the coarse levels (zeroBC = 1)
+    Bc tbc;      // residual pads read by the restriction
+    double tol;
+    int max_count;
+    int project;
+    const double *wc[3];
+    double wcsum;
+};
+
 }  // namespace sr

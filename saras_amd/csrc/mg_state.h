@@ -51,6 +51,12 @@ struct sr_mg {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_kind = -1;
+    // one-workgroup coarse end of the cycle (levels coarse_first..depth in LDS); first > depth = off
+    int coarse_first64 = 1 << 20, coarse_first32 = 1 << 20;
+    sr::CoarseDesc cdesc64{}, cdesc32{};
+    size_t clds64 = 0, clds32 = 0;
+    int coarse_enable = 1;
+    long lds_limit = 0;
     hipGraph_t graph_mixed_g[2] = {nullptr, nullptr};
     hipGraphExec_t graph_mixed[2] = {nullptr, nullptr};
     std::vector<double> history;

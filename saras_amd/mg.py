@@ -231,6 +231,11 @@ class Multigrid:
         _L.check(self.lib.sr_mg_time_kernel(self.h, int(kernel), int(level), int(reps), C.byref(out)))
         return out.value
 
+    def info(self, what):
+        v = C.c_long()
+        _L.check(self.lib.sr_mg_info(self.h, int(what), C.byref(v)))
+        return v.value
+
     def kernel_bytes(self, kernel, level):
         out = C.c_double()
         _L.check(self.lib.sr_mg_kernel_bytes(self.h, int(kernel), int(level), C.byref(out)))

@@ -303,3 +303,14 @@ def test_mixed_precision_dirichlet_r2():
     st = gm.mgSolve(p, f)
     assert st.converged
     assert np.max(np.abs(p[1:-1, 1:-1, 1:-1] - r2_exact(n))) < 1e-8
+
+
+def test_coarse_cycle_kernel_is_planned_and_matches_per_level_path():
+    """At 257^3 the coarse end of the cycle runs in one workgroup; on 33^3 the V-cycle that uses it
+    matches the oracle iterate for iterate (test_vcycle_iterates) -- here its plan is checked."""
+    pb = Problem(ind=(8, 8, 8), domain="PPN", mesh="UUD", depth=7)
+    g = pb.gpu()
+    first, lds, limit = g.info(0), g.info(2), g.info(4)
+    assert 1 <= first <= 7 and 0 < lds <= limit
+    pb2 = Problem(ind=(5, 5, 5), domain="PPN", mesh="UUD")
+    assert pb2.gpu().info(0) <= pb2.depth
