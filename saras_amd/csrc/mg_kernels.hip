@@ -47,6 +47,17 @@ __device__ inline unsigned fdivu(unsigned r, unsigned d, float inv) {
     return q;
 }
 
+// Grid-stride order for reductions over a whole level: the blocks of XCD x (blockIdx % 8) walk the
+// x-th contiguous eighth of the logical blocks, so the neighbour planes a block reads stay in that
+// XCD's L2.  Returns the first logical block, the stride and the end of this XCD's range.
+__device__ inline void xcd_range(unsigned total, unsigned &first, unsigned &stride, unsigned &end) {
+    unsigned x = blockIdx.x & 7u, w = blockIdx.x >> 3, per = gridDim.x >> 3;
+    unsigned chunk = (total + 7u) / 8u;
+    first = x * chunk + w;
+    stride = per;
+    end = min(total, (x + 1u) * chunk);
+}
+
 __device__ inline bool geo_slot(const Geo &g, unsigned b, unsigned &ip, unsigned &r) {
     unsigned p = b / g.bpp;
     r = (b - p * g.bpp) * 256u + threadIdx.x;
@@ -402,7 +413,9 @@ __global__ void __launch_bounds__(256) k_norm_partial(Lay L, Geo G, const T *__r
                                                       const T *__restrict__ cx, const T *__restrict__ cy,
                                                       const T *__restrict__ cz, double *__restrict__ part) {
     double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
-    for (unsigned lb = blockIdx.x; lb < G.total_blocks; lb += gridDim.x) {
+    unsigned lb0, lbs, lbe;
+    xcd_range(G.total_blocks, lb0, lbs, lbe);
+    for (unsigned lb = lb0; lb < lbe; lb += lbs) {
         unsigned ipu, r;
         if (!geo_slot(G, lb, ipu, r)) continue;
         unsigned jpu = fdivu(r, G.d1, G.inv1);
@@ -435,7 +448,9 @@ __global__ void __launch_bounds__(256) k_rhs_sums(Lay L, Geo G, const T *__restr
                                                   const double *__restrict__ wy, const double *__restrict__ wz,
                                                   double *__restrict__ part) {
     double s0 = 0, s1 = 0;
-    for (unsigned lb = blockIdx.x; lb < G.total_blocks; lb += gridDim.x) {
+    unsigned lb0, lbs, lbe;
+    xcd_range(G.total_blocks, lb0, lbs, lbe);
+    for (unsigned lb = lb0; lb < lbe; lb += lbs) {
         unsigned ipu, r;
         if (!geo_slot(G, lb, ipu, r)) continue;
         unsigned jpu = fdivu(r, G.d1, G.inv1);
@@ -621,7 +636,9 @@ __global__ void __launch_bounds__(256) k_outer_residual(Lay L, Geo G, const doub
                                                         double *__restrict__ part) {
     double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
     const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
-    for (unsigned lb = blockIdx.x; lb < G.total_blocks; lb += gridDim.x) {
+    unsigned lb0, lbs, lbe;
+    xcd_range(G.total_blocks, lb0, lbs, lbe);
+    for (unsigned lb = lb0; lb < lbe; lb += lbs) {
         unsigned ipu, r;
         if (!geo_slot(G, lb, ipu, r)) continue;
         unsigned jpu = fdivu(r, G.d1, G.inv1);
@@ -925,7 +942,11 @@ static Geo geo_core(const Lay &L) { return geo_planes((unsigned)(L.ny * 2 * L.kh
 // every stored slot of the padded level
 static Geo geo_all(const Lay &L) { return geo_planes((unsigned)((L.ny + 2) * 2 * L.khs), 0, L.nx + 2, 2 * L.khs); }
 static const unsigned REDUCE_BLOCKS = 2048;
-static unsigned reduce_grid(const Geo &g) { return g.total_blocks < REDUCE_BLOCKS ? g.total_blocks : REDUCE_BLOCKS; }
+// a multiple of 8 (one share per XCD), at most REDUCE_BLOCKS
+static unsigned reduce_grid(const Geo &g) {
+    unsigned n = g.total_blocks < REDUCE_BLOCKS ? g.total_blocks : REDUCE_BLOCKS;
+    return (n + 7u) & ~7u;
+}
 
 template <typename T>
 void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s) {
