@@ -1,0 +1,87 @@
+/*
+ * saras_ts.h -- C ABI of the MI355X hydro time step: the field stencils, the implicit Jacobi
+ * viscous solve and the pressure projection that src/solvers/hydro/hydro_d3.cc runs every step,
+ * with the multigrid pressure solve of saras_mg.h inside.
+ *
+ * Fields use the reference's padded layout (C order, k fastest, index -1 first):
+ *   Vx: (Nx-1+2) x (Ny+2) x (Nz+2)   (x faces, y/z nodes; lib/field/vfield.cc:64-66)
+ *   Vy: (Nx+2) x (Ny-1+2) x (Nz+2),  Vz: (Nx+2) x (Ny+2) x (Nz-1+2)
+ *   P, Pp, mgRHS, divergence: (Nx+2) x (Ny+2) x (Nz+2)
+ * All fields live in HBM for the life of the handle.  Status codes as in saras_mg.h.
+ */
+#ifndef SARAS_TS_H
+#define SARAS_TS_H
+
+#include "saras_mg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* which field */
+#define SR_TS_VX 0
+#define SR_TS_VY 1
+#define SR_TS_VZ 2
+#define SR_TS_P 3
+#define SR_TS_PP 4          /* pressure correction (the multigrid solution, warm start) */
+#define SR_TS_MGRHS 5
+#define SR_TS_HX 6          /* nseRHS components */
+#define SR_TS_HY 7
+#define SR_TS_HZ 8
+#define SR_TS_DIV 12
+
+typedef struct sr_ts sr_ts;
+
+/* The parser fields the time step reads (lib/io/parser.h:65-102). */
+typedef struct sr_ts_desc {
+    int xInd, yInd, zInd;
+    double Lx, Ly, Lz;
+    int xGrid, yGrid, zGrid;        /* 1 uniform, 2 tanh */
+    double betaX, betaY, betaZ;
+    int xPer, yPer, zPer;
+    int probType;                   /* velocity walls of hydro::initVBC (hydro.cc:287-371) */
+    double Re, dt, cnTolerance;
+    int vcDepth, vcCount, preSmooth, postSmooth, resType, gsSmooth;
+    double mgTolerance;
+} sr_ts_desc;
+
+typedef struct sr_ts_stats {
+    int jacobi_iters[3];            /* sweeps of solveVx / solveVy / solveVz                       */
+    int mg_cycles;                  /* V-cycles of the pressure solve                              */
+    double step_ms;                 /* wall time of the call                                       */
+} sr_ts_stats;
+
+/* hydro_d3::hydro_d3 (src/solvers/hydro/hydro_d3.cc:63-119) without the initial condition */
+int sr_ts_create(const sr_ts_desc *desc, sr_ts **out);
+int sr_ts_destroy(sr_ts *ts);
+/* P = 1 and the Taylor-Green velocity (lib/initial/taylorGreen.cc:54-100), then the velocity BCs */
+int sr_ts_init_tgv(sr_ts *ts);
+/* host <-> device copies of a whole padded field */
+int sr_ts_upload(sr_ts *ts, int which, const double *host);
+int sr_ts_download(sr_ts *ts, int which, double *host);
+int sr_ts_field_dims(const sr_ts *ts, int which, int *nx, int *ny, int *nz);   /* core sizes */
+
+/* hydro_d3::timeAdvance (hydro_d3.cc:254-368), zero forcing */
+int sr_ts_time_advance(sr_ts *ts, sr_ts_stats *stats);
+
+/* ---- pieces of the step, for parity tests ------------------------------------------------- */
+/* derivative::calcDerivative1_{x,y,z} / calcDerivative2{xx,yy,zz} (lib/field/derivative.cc:121-243)
+ * of field `which` (a velocity component or P) along `dim`; out is a host array of that field's
+ * padded shape, written over the core range along dim and every index of the other two.           */
+int sr_ts_deriv(sr_ts *ts, int which, int order, int dim, double *out);
+/* nseRHS = V + dt (computeDiff / Re - computeNLin - grad P), hydro_d3.cc:259-303 (core values) */
+int sr_ts_build_rhs(sr_ts *ts);
+/* solveVx / solveVy / solveVz (hydro_d3.cc:370-533); iterations returned in *iters              */
+int sr_ts_solve_v(sr_ts *ts, int component, int *iters);
+/* hydro::imposeU/V/WBCs (hydro.cc:386-461) */
+int sr_ts_impose_vbc(sr_ts *ts, int component);
+/* vfield::divergence into SR_TS_DIV (vfield.cc:299-320) */
+int sr_ts_divergence(sr_ts *ts);
+/* tseries energy and maximum divergence (lib/io/tseries.cc:137-178) */
+int sr_ts_energy(sr_ts *ts, double *energy);
+int sr_ts_max_div(sr_ts *ts, double *maxdiv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -106,6 +106,43 @@ const double *or_mg_metric(const or_mg *m, int which, int level);
 double or_mg_coef(const or_mg *m, int which, int level);  /* 0 hx 1 hy 2 hz 3 hxhy 4 hyhz 5 hzhx 6 hxhyhz */
 int    or_mg_max_count(const or_mg *m);
 
+/* ---------- hydro time step (field_oracle.c) ---------- */
+typedef struct or_ts_params {
+    int xInd, yInd, zInd;
+    double Lx, Ly, Lz;
+    int xGrid, yGrid, zGrid;
+    double betaX, betaY, betaZ;
+    int xPer, yPer, zPer;
+    int probType;
+    double Re, dt, cnTolerance;
+    int vcDepth, vcCount, preSmooth, postSmooth, resType, smoother;
+    double mgTolerance;
+} or_ts_params;
+
+typedef struct or_ts or_ts;
+
+or_ts *or_ts_create(const or_ts_params *p);
+void   or_ts_destroy(or_ts *t);
+/* which: 0..2 Vx Vy Vz, 3 P, 4 Pp, 5 mgRHS, 6..8 nseRHS, 9..11 pressure gradient, 12 divergence */
+double *or_ts_array(or_ts *t, int which);
+void   or_ts_shape(const or_ts *t, int which, int *nx, int *ny, int *nz);
+/* dim 0..2, node 1 = staggered / 0 = collocated, quantity 0 x, 1 xi_x, 2 xi_xx, 3 xi_x^2 */
+const double *or_ts_metric(const or_ts *t, int dim, int node, int quantity);
+void   or_ts_deriv(or_ts *t, int which, int order, int dim, double *out);
+void   or_ts_compute_diff(or_ts *t);
+void   or_ts_compute_nlin(or_ts *t);
+void   or_ts_gradient(or_ts *t, int from_pp);
+void   or_ts_divergence(or_ts *t);
+void   or_ts_impose_vbc(or_ts *t, int component);
+void   or_ts_init_tgv(or_ts *t);
+double or_ts_energy(const or_ts *t);
+double or_ts_max_div(or_ts *t);
+int    or_ts_solve_v(or_ts *t, int component);
+void   or_ts_build_rhs(or_ts *t);
+int    or_ts_time_advance(or_ts *t);
+int    or_ts_jacobi_iters(const or_ts *t, int component);
+int    or_ts_mg_cycles(const or_ts *t);
+
 #ifdef __cplusplus
 }
 #endif

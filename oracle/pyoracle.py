@@ -215,3 +215,155 @@ def null_vector_1d(n, periodic, h, d2, d1):
     d1 = np.ascontiguousarray(d1, dtype=np.float64)
     L.or_null_vector_1d(n, int(periodic), float(h), _ptr(d2), _ptr(d1), _ptr(w))
     return w
+
+
+# ------------------------------------------------------------------------------------------------
+# hydro time step (field_oracle.c)
+# ------------------------------------------------------------------------------------------------
+class TsParams(C.Structure):
+    _fields_ = [
+        ("xInd", C.c_int), ("yInd", C.c_int), ("zInd", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+        ("xGrid", C.c_int), ("yGrid", C.c_int), ("zGrid", C.c_int),
+        ("betaX", C.c_double), ("betaY", C.c_double), ("betaZ", C.c_double),
+        ("xPer", C.c_int), ("yPer", C.c_int), ("zPer", C.c_int),
+        ("probType", C.c_int),
+        ("Re", C.c_double), ("dt", C.c_double), ("cnTolerance", C.c_double),
+        ("vcDepth", C.c_int), ("vcCount", C.c_int), ("preSmooth", C.c_int), ("postSmooth", C.c_int),
+        ("resType", C.c_int), ("smoother", C.c_int),
+        ("mgTolerance", C.c_double),
+    ]
+
+
+VX, VY, VZ, PRES, PCORR, MGRHS, HX, HY, HZ, GX, GY, GZ, DIV = range(13)
+
+
+def _ts_bind(L):
+    if getattr(L, "_ts_bound", False):
+        return
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.or_ts_create.restype = vp
+    L.or_ts_create.argtypes = [C.POINTER(TsParams)]
+    L.or_ts_destroy.argtypes = [vp]
+    L.or_ts_array.restype = dp
+    L.or_ts_array.argtypes = [vp, C.c_int]
+    L.or_ts_shape.argtypes = [vp, C.c_int, ip, ip, ip]
+    L.or_ts_metric.restype = dp
+    L.or_ts_metric.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.or_ts_deriv.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    for f in ("or_ts_compute_diff", "or_ts_compute_nlin", "or_ts_divergence", "or_ts_init_tgv",
+              "or_ts_build_rhs"):
+        getattr(L, f).argtypes = [vp]
+    L.or_ts_gradient.argtypes = [vp, C.c_int]
+    L.or_ts_impose_vbc.argtypes = [vp, C.c_int]
+    L.or_ts_energy.restype = C.c_double
+    L.or_ts_energy.argtypes = [vp]
+    L.or_ts_max_div.restype = C.c_double
+    L.or_ts_max_div.argtypes = [vp]
+    L.or_ts_solve_v.restype = C.c_int
+    L.or_ts_solve_v.argtypes = [vp, C.c_int]
+    L.or_ts_time_advance.restype = C.c_int
+    L.or_ts_time_advance.argtypes = [vp]
+    L.or_ts_jacobi_iters.restype = C.c_int
+    L.or_ts_jacobi_iters.argtypes = [vp, C.c_int]
+    L.or_ts_mg_cycles.restype = C.c_int
+    L.or_ts_mg_cycles.argtypes = [vp]
+    L._ts_bound = True
+
+
+def ts_params(ind=(5, 5, 5), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1.0, 1.0, 1.0), Re=1600.0,
+              dt=1e-3, cn_tol=1e-6, depth=None, vc_count=5, pre=4, post=4, res_type=1, smoother=SMOOTH_RB,
+              mg_tol=1e-6, prob_type=2):
+    p = TsParams()
+    p.xInd, p.yInd, p.zInd = ind
+    p.Lx, p.Ly, p.Lz = L
+    p.xGrid, p.yGrid, p.zGrid = (1 if c == "U" else 2 for c in mesh)
+    p.betaX, p.betaY, p.betaZ = beta
+    p.xPer, p.yPer, p.zPer = (int(c == "P") for c in domain)
+    p.probType = prob_type
+    p.Re, p.dt, p.cnTolerance = Re, dt, cn_tol
+    p.vcDepth = depth if depth is not None else min(ind) - 1
+    p.vcCount, p.preSmooth, p.postSmooth, p.resType, p.smoother = vc_count, pre, post, res_type, smoother
+    p.mgTolerance = mg_tol
+    return p
+
+
+class OracleTS:
+    """The restated hydro_d3 time step (fields, stencils, Jacobi viscous solve, projection)."""
+
+    def __init__(self, params, fast=False):
+        self.L = lib(fast)
+        _ts_bind(self.L)
+        self.params = params
+        self.h = self.L.or_ts_create(C.byref(params))
+
+    def close(self):
+        if self.h:
+            self.L.or_ts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, which):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self.L.or_ts_shape(self.h, which, C.byref(a), C.byref(b), C.byref(c))
+        return a.value + 2, b.value + 2, c.value + 2
+
+    def array(self, which):
+        return np.ctypeslib.as_array(self.L.or_ts_array(self.h, which), shape=self.shape(which))
+
+    def metric(self, dim, node, quantity):
+        n = (1 << (self.params.xInd, self.params.yInd, self.params.zInd)[dim]) + 1
+        m = n + 2 if node else n + 1
+        return np.ctypeslib.as_array(self.L.or_ts_metric(self.h, dim, int(node), quantity), shape=(m,)).copy()
+
+    def deriv(self, which, order, dim):
+        out = np.zeros(self.shape(which))
+        self.L.or_ts_deriv(self.h, which, order, dim, _ptr(out))
+        return out
+
+    def init_tgv(self):
+        self.L.or_ts_init_tgv(self.h)
+
+    def compute_diff(self):
+        self.L.or_ts_compute_diff(self.h)
+
+    def compute_nlin(self):
+        self.L.or_ts_compute_nlin(self.h)
+
+    def gradient(self, from_pp=False):
+        self.L.or_ts_gradient(self.h, int(from_pp))
+
+    def divergence(self):
+        self.L.or_ts_divergence(self.h)
+
+    def impose_vbc(self, c):
+        self.L.or_ts_impose_vbc(self.h, c)
+
+    def build_rhs(self):
+        self.L.or_ts_build_rhs(self.h)
+
+    def solve_v(self, c):
+        return self.L.or_ts_solve_v(self.h, c)
+
+    def time_advance(self):
+        r = self.L.or_ts_time_advance(self.h)
+        if r != 0:
+            raise RuntimeError("Jacobi iterations did not converge")
+
+    def energy(self):
+        return self.L.or_ts_energy(self.h)
+
+    def max_div(self):
+        return self.L.or_ts_max_div(self.h)
+
+    def jacobi_iters(self, c):
+        return self.L.or_ts_jacobi_iters(self.h, c)
+
+    def mg_cycles(self):
+        return self.L.or_ts_mg_cycles(self.h)

@@ -1,4 +1,4 @@
-"""ctypes binding of the C ABI in include/saras_mg.h (libsaras_mg.so, gfx950).
+"""ctypes binding of the C ABI in include/saras_mg.h and include/saras_ts.h (libsaras_mg.so, gfx950).
 
 The library is loaded from saras_amd/_build; a missing library is an error (there is no CPU
 fallback for any solver entry point)."""
@@ -37,7 +37,30 @@ class SrMgStats(C.Structure):
                 ("sweep0_ms", C.c_double)]
 
 
-# (name, restype, argtypes) of every entry point declared in include/saras_mg.h
+SR_TS_VX, SR_TS_VY, SR_TS_VZ, SR_TS_P, SR_TS_PP, SR_TS_MGRHS = 0, 1, 2, 3, 4, 5
+SR_TS_HX, SR_TS_HY, SR_TS_HZ, SR_TS_DIV = 6, 7, 8, 12
+
+
+class SrTsDesc(C.Structure):
+    _fields_ = [
+        ("xInd", C.c_int), ("yInd", C.c_int), ("zInd", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+        ("xGrid", C.c_int), ("yGrid", C.c_int), ("zGrid", C.c_int),
+        ("betaX", C.c_double), ("betaY", C.c_double), ("betaZ", C.c_double),
+        ("xPer", C.c_int), ("yPer", C.c_int), ("zPer", C.c_int),
+        ("probType", C.c_int),
+        ("Re", C.c_double), ("dt", C.c_double), ("cnTolerance", C.c_double),
+        ("vcDepth", C.c_int), ("vcCount", C.c_int), ("preSmooth", C.c_int), ("postSmooth", C.c_int),
+        ("resType", C.c_int), ("gsSmooth", C.c_int),
+        ("mgTolerance", C.c_double),
+    ]
+
+
+class SrTsStats(C.Structure):
+    _fields_ = [("jacobi_iters", C.c_int * 3), ("mg_cycles", C.c_int), ("step_ms", C.c_double)]
+
+
+# (name, restype, argtypes) of every entry point declared in include/saras_mg.h and saras_ts.h
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
 SIGNATURES = [
@@ -68,6 +91,20 @@ SIGNATURES = [
     ("sr_device_free", C.c_int, [C.c_void_p]),
     ("sr_memcpy", C.c_int, [C.c_void_p, C.c_void_p, C.c_ulong, C.c_int]),
     ("sr_last_error", C.c_char_p, []),
+    ("sr_ts_create", C.c_int, [C.POINTER(SrTsDesc), C.POINTER(C.c_void_p)]),
+    ("sr_ts_destroy", C.c_int, [_vp]),
+    ("sr_ts_init_tgv", C.c_int, [_vp]),
+    ("sr_ts_upload", C.c_int, [_vp, C.c_int, _dp]),
+    ("sr_ts_download", C.c_int, [_vp, C.c_int, _dp]),
+    ("sr_ts_field_dims", C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
+    ("sr_ts_time_advance", C.c_int, [_vp, C.POINTER(SrTsStats)]),
+    ("sr_ts_deriv", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    ("sr_ts_build_rhs", C.c_int, [_vp]),
+    ("sr_ts_solve_v", C.c_int, [_vp, C.c_int, _ip]),
+    ("sr_ts_impose_vbc", C.c_int, [_vp, C.c_int]),
+    ("sr_ts_divergence", C.c_int, [_vp]),
+    ("sr_ts_energy", C.c_int, [_vp, _dp]),
+    ("sr_ts_max_div", C.c_int, [_vp, _dp]),
 ]
 
 _lib = None

@@ -1,0 +1,534 @@
+// C ABI of the hydro time step (include/saras_ts.h): hydro_d3::timeAdvance
+// (src/solvers/hydro/hydro_d3.cc:254-368) with every field resident in HBM and the multigrid
+// pressure solve of saras_mg.h inside.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/saras_ts.h"
+#include "mg_state.h"
+#include "ts_kernels.h"
+
+using namespace sr;
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess)                                                                         \
+            return set_err(SR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+#define RETCHK(expr)                                                                                  \
+    do {                                                                                              \
+        int _r = (expr);                                                                              \
+        if (_r != SR_OK) return _r;                                                                   \
+    } while (0)
+
+struct sr_ts {
+    sr_ts_desc d{};
+    int N[3] = {0, 0, 0};
+    double h[3] = {0, 0, 0};
+    FShape fp{}, fv[3]{};
+    std::vector<double> hmet[3][2][4];   // host metrics, [dim][node 0 / face 1][quantity]
+    double *dmet = nullptr;
+    Metrics M{};
+    double *V[3] = {nullptr, nullptr, nullptr};
+    double *H[3] = {nullptr, nullptr, nullptr};
+    double *G[3] = {nullptr, nullptr, nullptr};   // Jacobi target, swapped with V after each sweep
+    double *P = nullptr, *Pp = nullptr, *mgRHS = nullptr, *divV = nullptr;
+    double *scratch = nullptr;                    // derivative output of the parity entry point
+    double *part = nullptr, *dred = nullptr, *hred = nullptr;
+    unsigned long long *dmax = nullptr, *hmax = nullptr;
+    std::vector<GhostWave> vbc[3];               // imposeU/V/WBCs as ordered waves
+    GhostWave sync_v[3]{}, sync_p{};             // syncData of the velocity components and of P
+    sr_mg *mg = nullptr;
+    hipStream_t s = nullptr;
+    int jac[3] = {0, 0, 0};
+    int mg_cycles = 0;
+};
+
+static long fsize(const FShape &f) { return (long)(f.n[0] + 2) * (f.n[1] + 2) * (f.n[2] + 2); }
+
+// ------------------------------------------------------------------------------------------------
+// ghost waves
+// ------------------------------------------------------------------------------------------------
+static void add_op(GhostWave &w, const GhostOp &op) {
+    if (op.count > 0) w.ops[w.nops++] = op;
+}
+
+// mpidata::syncData (mpidata.cc:278-306) when the neighbour along periodic x / y is the rank itself:
+// the faces over the core of the other two directions, then the four x-y edges over the z core.
+// The edge exchange runs whatever the periodicity (edge ranks are never cleared, hydro.cc:174-212).
+static GhostWave sync_wave(const FShape &f, int xPer, int yPer) {
+    GhostWave w{};
+    int nx = f.n[0], ny = f.n[1], nz = f.n[2], sx = f.s[0], sy = f.s[1];
+    auto plane = [&](int dim, int dst, int src, int alo, int ahi, int blo, int bhi) {
+        GhostOp o{};
+        o.kind = 0;
+        o.dim = dim;
+        o.dst[0] = dst;
+        o.src[0] = src;
+        o.lo[0] = alo; o.hi[0] = ahi;
+        o.lo[1] = blo; o.hi[1] = bhi;
+        o.a = 1.0; o.b = 0.0;
+        o.count = (long)(ahi - alo + 1) * (bhi - blo + 1);
+        add_op(w, o);
+    };
+    auto line = [&](int dx, int dy, int sxv, int syv) {
+        GhostOp o{};
+        o.kind = 1;
+        o.dim = 2;
+        o.dst[0] = dx; o.dst[1] = dy;
+        o.src[0] = sxv; o.src[1] = syv;
+        o.lo[0] = 0; o.hi[0] = nz - 1;
+        o.a = 1.0; o.b = 0.0;
+        o.count = nz;
+        add_op(w, o);
+    };
+    if (xPer) {
+        plane(0, -1, nx - 1 - sx, 0, ny - 1, 0, nz - 1);
+        plane(0, nx, sx, 0, ny - 1, 0, nz - 1);
+    }
+    if (yPer) {
+        plane(1, -1, ny - 1 - sy, 0, nx - 1, 0, nz - 1);
+        plane(1, ny, sy, 0, nx - 1, 0, nz - 1);
+    }
+    line(-1, -1, nx - 1 - sx, ny - 1 - sy);
+    line(-1, ny, nx - 1 - sx, sy);
+    line(nx, -1, sx, ny - 1 - sy);
+    line(nx, ny, sx, sy);
+    return w;
+}
+
+// one wall of a field (lib/boundary/*.cc): the wall slice is the layer just outside the bulk
+// (field.cc:180-290).  kind 0 periodicCC, 1 periodicFC, 2 dirichletCC, 3 dirichletFC,
+// 4 neumannCC, 5 neumannFC.
+static GhostOp wall_op(const FShape &f, int per, int wall, int kind, double value) {
+    int d = wall / 2, hi = wall & 1;
+    int n = f.n[d];
+    int blo = 0, bhi = n - 1;
+    if (f.s[d] && !per) {
+        blo = 1;
+        bhi = n - 2;
+    }
+    int w = hi ? bhi + 1 : blo - 1;
+    int wo = hi ? blo - 1 : bhi + 1;
+    int src;
+    switch (kind) {
+        case 0: src = hi ? wo + 2 : wo - 2; break;
+        case 1: src = hi ? wo + 1 : wo - 1; break;
+        case 3: case 4: case 5: src = hi ? w - 1 : w + 1; break;
+        default: src = w; break;
+    }
+    GhostOp o{};
+    o.kind = 0;
+    o.dim = d;
+    o.dst[0] = w;
+    o.src[0] = src;
+    int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+    o.lo[0] = -1; o.hi[0] = f.n[a];
+    o.lo[1] = -1; o.hi[1] = f.n[b];
+    o.count = (long)(f.n[a] + 2) * (f.n[b] + 2);
+    switch (kind) {
+        case 2: o.a = 0.0; o.b = value; break;
+        case 3: o.a = -1.0; o.b = 2.0 * value; break;     // 2 v - F(wall +- 1)
+        default: o.a = 1.0; o.b = 0.0; break;
+    }
+    return o;
+}
+
+// hydro::imposeU/V/WBCs (hydro.cc:386-461) with the walls of hydro::initVBC (hydro.cc:287-371):
+// syncData, then x walls, y walls, and the top and bottom walls.  Walls normal to the component are
+// no-penetration (dirichletFC 0), the others no-slip (dirichletCC 0; the lid of probType 1 moves
+// Vx at 1); periodicCC / periodicFC in z when zPer.
+static std::vector<GhostWave> vbc_waves(const sr_ts_desc &d, const FShape &f, int c) {
+    std::vector<GhostWave> ws;
+    ws.push_back(sync_wave(f, d.xPer, d.yPer));
+    int per[3] = {d.xPer, d.yPer, d.zPer};
+    for (int dim = 0; dim < 2; dim++) {
+        if (per[dim]) continue;
+        int kind = dim == c ? 3 : 2;
+        GhostWave w{};
+        add_op(w, wall_op(f, 0, 2 * dim, kind, 0.0));
+        add_op(w, wall_op(f, 0, 2 * dim + 1, kind, 0.0));
+        ws.push_back(w);
+    }
+    GhostWave w{};
+    if (per[2]) {
+        int kind = c == 2 ? 1 : 0;
+        add_op(w, wall_op(f, 1, 5, kind, 0.0));
+        add_op(w, wall_op(f, 1, 4, kind, 0.0));
+    } else {
+        int kind = c == 2 ? 3 : 2;
+        double top = (d.probType == 1 && c == 0) ? 1.0 : 0.0;
+        add_op(w, wall_op(f, 0, 5, kind, top));
+        add_op(w, wall_op(f, 0, 4, kind, 0.0));
+    }
+    ws.push_back(w);
+    return ws;
+}
+
+static void impose_vbc(sr_ts *t, int c) {
+    for (const GhostWave &w : t->vbc[c]) ts_ghost_wave(t->fv[c], t->V[c], w, t->s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// construction
+// ------------------------------------------------------------------------------------------------
+static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
+    t->d = *desc;
+    const sr_ts_desc &d = t->d;
+    int ind[3] = {d.xInd, d.yInd, d.zInd};
+    double L[3] = {d.Lx, d.Ly, d.Lz};
+    int gt[3] = {d.xGrid, d.yGrid, d.zGrid};
+    double beta[3] = {d.betaX, d.betaY, d.betaZ};
+    for (int k = 0; k < 3; k++) {
+        if (ind[k] < 2 || ind[k] > 12) return set_err(SR_ERR_ARG, "grid size index out of range 2..12");
+        if (L[k] <= 0.0) return set_err(SR_ERR_ARG, "domain length must be positive");
+    }
+    if (d.Re <= 0.0 || d.dt <= 0.0 || d.cnTolerance <= 0.0)
+        return set_err(SR_ERR_ARG, "Re, dt and cnTolerance must be positive");
+    for (int k = 0; k < 3; k++) {
+        int n = (1 << ind[k]) + 1;
+        t->N[k] = n;
+        t->h[k] = 1.0 / (double)(n - 1);
+        for (int face = 0; face < 2; face++)
+            for (int q = 0; q < 4; q++) t->hmet[k][face][q].assign((size_t)(face ? n + 1 : n + 2), 0.0);
+        for (int face = 0; face < 2; face++) {
+            int r = sr_grid_axis(n, L[k], gt[k], beta[k], face, t->hmet[k][face][0].data(), t->hmet[k][face][1].data(),
+                                 t->hmet[k][face][2].data(), t->hmet[k][face][3].data());
+            if (r != SR_OK) return set_err(r, "invalid grid description");
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        t->fp.n[k] = t->N[k];
+        t->fp.s[k] = 1;
+    }
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < 3; k++) {
+            t->fv[c].n[k] = c == k ? t->N[k] - 1 : t->N[k];
+            t->fv[c].s[k] = c == k ? 0 : 1;
+        }
+    // metric arrays in one allocation
+    size_t tot = 0;
+    for (int k = 0; k < 3; k++)
+        for (int face = 0; face < 2; face++)
+            for (int q = 0; q < 4; q++) tot += t->hmet[k][face][q].size();
+    HIPCHK(hipMalloc((void **)&t->dmet, sizeof(double) * tot));
+    size_t off = 0;
+    for (int k = 0; k < 3; k++)
+        for (int face = 0; face < 2; face++)
+            for (int q = 0; q < 4; q++) {
+                const std::vector<double> &v = t->hmet[k][face][q];
+                HIPCHK(hipMemcpy(t->dmet + off, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+                t->M.m[k][face][q] = t->dmet + off;
+                off += v.size();
+            }
+    // the pressure solver on the node grid (hydro_d3.cc:63-66)
+    sr_mg_desc md;
+    std::memset(&md, 0, sizeof(md));
+    md.xInd = d.xInd; md.yInd = d.yInd; md.zInd = d.zInd;
+    md.vcDepth = d.vcDepth; md.vcCount = d.vcCount;
+    md.preSmooth = d.preSmooth; md.postSmooth = d.postSmooth;
+    md.gsSmooth = d.gsSmooth; md.resType = d.resType;
+    md.xPer = d.xPer; md.yPer = d.yPer; md.zPer = d.zPer;
+    md.semantics = SR_SEM_REFERENCE;
+    md.precision = SR_PREC_FP64;
+    md.mgTolerance = d.mgTolerance;
+    md.relTolerance = 1e-10;
+    md.dXi = t->h[0]; md.dEt = t->h[1]; md.dZt = t->h[2];
+    md.xix2 = t->hmet[0][0][3].data(); md.xixx = t->hmet[0][0][2].data();
+    md.ety2 = t->hmet[1][0][3].data(); md.etyy = t->hmet[1][0][2].data();
+    md.ztz2 = t->hmet[2][0][3].data(); md.ztzz = t->hmet[2][0][2].data();
+    RETCHK(sr_mg_create(&md, &t->mg));
+    t->s = t->mg->stream;
+    // fields
+    long np = fsize(t->fp);
+    HIPCHK(hipMalloc((void **)&t->P, sizeof(double) * np));
+    HIPCHK(hipMalloc((void **)&t->Pp, sizeof(double) * np));
+    HIPCHK(hipMalloc((void **)&t->mgRHS, sizeof(double) * np));
+    HIPCHK(hipMalloc((void **)&t->divV, sizeof(double) * np));
+    long nmax = np;
+    for (int c = 0; c < 3; c++) {
+        long nv = fsize(t->fv[c]);
+        nmax = std::max(nmax, nv);
+        HIPCHK(hipMalloc((void **)&t->V[c], sizeof(double) * nv));
+        HIPCHK(hipMalloc((void **)&t->H[c], sizeof(double) * nv));
+        HIPCHK(hipMalloc((void **)&t->G[c], sizeof(double) * nv));
+        ts_fill(t->V[c], nv, 0.0, t->s);
+        ts_fill(t->H[c], nv, 0.0, t->s);
+        ts_fill(t->G[c], nv, 0.0, t->s);
+    }
+    HIPCHK(hipMalloc((void **)&t->scratch, sizeof(double) * nmax));
+    ts_fill(t->P, np, 0.0, t->s);
+    ts_fill(t->Pp, np, 0.0, t->s);
+    ts_fill(t->mgRHS, np, 0.0, t->s);
+    ts_fill(t->divV, np, 0.0, t->s);
+    HIPCHK(hipMalloc((void **)&t->part, sizeof(double) * 2 * (size_t)ts_energy_parts(t->fp)));
+    HIPCHK(hipMalloc((void **)&t->dred, sizeof(double) * 4));
+    HIPCHK(hipHostMalloc((void **)&t->hred, sizeof(double) * 4, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&t->dmax, sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc((void **)&t->hmax, sizeof(unsigned long long), hipHostMallocDefault));
+    for (int c = 0; c < 3; c++) {
+        t->vbc[c] = vbc_waves(d, t->fv[c], c);
+        t->sync_v[c] = sync_wave(t->fv[c], d.xPer, d.yPer);
+    }
+    t->sync_p = sync_wave(t->fp, d.xPer, d.yPer);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_destroy(sr_ts *t) {
+    if (!t) return SR_OK;
+    if (t->s) (void)hipStreamSynchronize(t->s);
+    double *bufs[] = {t->dmet, t->P, t->Pp, t->mgRHS, t->divV, t->scratch, t->part, t->dred};
+    for (double *b : bufs)
+        if (b) (void)hipFree(b);
+    for (int c = 0; c < 3; c++) {
+        if (t->V[c]) (void)hipFree(t->V[c]);
+        if (t->H[c]) (void)hipFree(t->H[c]);
+        if (t->G[c]) (void)hipFree(t->G[c]);
+    }
+    if (t->dmax) (void)hipFree(t->dmax);
+    if (t->hred) (void)hipHostFree(t->hred);
+    if (t->hmax) (void)hipHostFree(t->hmax);
+    if (t->mg) sr_mg_destroy(t->mg);
+    delete t;
+    return SR_OK;
+}
+
+extern "C" int sr_ts_create(const sr_ts_desc *desc, sr_ts **out) {
+    if (!desc || !out) return set_err(SR_ERR_ARG, "null argument");
+    *out = nullptr;
+    sr_ts *t = new sr_ts();
+    int r = create_impl(desc, t);
+    if (r != SR_OK) {
+        std::string msg = sr_last_error();
+        sr_ts_destroy(t);
+        return set_err(r, msg);
+    }
+    *out = t;
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// field access
+// ------------------------------------------------------------------------------------------------
+static int field(sr_ts *t, int which, double **ptr, const FShape **shape) {
+    switch (which) {
+        case SR_TS_VX: case SR_TS_VY: case SR_TS_VZ:
+            *ptr = t->V[which]; *shape = &t->fv[which]; return SR_OK;
+        case SR_TS_P: *ptr = t->P; *shape = &t->fp; return SR_OK;
+        case SR_TS_PP: *ptr = t->Pp; *shape = &t->fp; return SR_OK;
+        case SR_TS_MGRHS: *ptr = t->mgRHS; *shape = &t->fp; return SR_OK;
+        case SR_TS_HX: case SR_TS_HY: case SR_TS_HZ:
+            *ptr = t->H[which - SR_TS_HX]; *shape = &t->fv[which - SR_TS_HX]; return SR_OK;
+        case SR_TS_DIV: *ptr = t->divV; *shape = &t->fp; return SR_OK;
+        default: return set_err(SR_ERR_ARG, "unknown field");
+    }
+}
+
+extern "C" int sr_ts_field_dims(const sr_ts *t, int which, int *nx, int *ny, int *nz) {
+    if (!t || !nx || !ny || !nz) return set_err(SR_ERR_ARG, "null argument");
+    double *p;
+    const FShape *f;
+    RETCHK(field(const_cast<sr_ts *>(t), which, &p, &f));
+    *nx = f->n[0]; *ny = f->n[1]; *nz = f->n[2];
+    return SR_OK;
+}
+
+extern "C" int sr_ts_upload(sr_ts *t, int which, const double *host) {
+    if (!t || !host) return set_err(SR_ERR_ARG, "null argument");
+    double *p;
+    const FShape *f;
+    RETCHK(field(t, which, &p, &f));
+    HIPCHK(hipMemcpyAsync(p, host, sizeof(double) * fsize(*f), hipMemcpyHostToDevice, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_download(sr_ts *t, int which, double *host) {
+    if (!t || !host) return set_err(SR_ERR_ARG, "null argument");
+    double *p;
+    const FShape *f;
+    RETCHK(field(t, which, &p, &f));
+    HIPCHK(hipMemcpyAsync(host, p, sizeof(double) * fsize(*f), hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// pieces of the step
+// ------------------------------------------------------------------------------------------------
+extern "C" int sr_ts_init_tgv(sr_ts *t) {
+    if (!t) return set_err(SR_ERR_ARG, "null argument");
+    ts_fill(t->P, fsize(t->fp), 1.0, t->s);
+    ts_fill(t->Pp, fsize(t->fp), 0.0, t->s);
+    for (int c = 0; c < 3; c++) ts_tgv(t->fv[c], t->M, c, t->d.Lx, t->d.Ly, t->d.Lz, t->V[c], t->s);
+    for (int c = 0; c < 3; c++) impose_vbc(t, c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_impose_vbc(sr_ts *t, int component) {
+    if (!t || component < 0 || component > 2) return set_err(SR_ERR_ARG, "bad component");
+    impose_vbc(t, component);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_deriv(sr_ts *t, int which, int order, int dim, double *out) {
+    if (!t || !out || (order != 1 && order != 2) || dim < 0 || dim > 2) return set_err(SR_ERR_ARG, "bad argument");
+    if (which != SR_TS_VX && which != SR_TS_VY && which != SR_TS_VZ && which != SR_TS_P)
+        return set_err(SR_ERR_ARG, "derivatives are taken of a velocity component or P");
+    double *p;
+    const FShape *f;
+    RETCHK(field(t, which, &p, &f));
+    ts_fill(t->scratch, fsize(*f), 0.0, t->s);
+    ts_deriv(*f, t->M, p, t->scratch, order, dim, t->h[dim], t->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, t->scratch, sizeof(double) * fsize(*f), hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+static void build_rhs(sr_ts *t) {
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    for (int c = 0; c < 3; c++) {
+        ts_build_rhs(c, t->fv, t->fp, t->M, Vc, t->P, t->H[c], t->h, 1.0 / t->d.Re, t->d.dt, t->s);
+        ts_ghost_wave(t->fv[c], t->H[c], t->sync_v[c], t->s);
+    }
+}
+
+extern "C" int sr_ts_build_rhs(sr_ts *t) {
+    if (!t) return set_err(SR_ERR_ARG, "null argument");
+    build_rhs(t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+// hydro_d3::solveVx/Vy/Vz (hydro_d3.cc:370-533): Jacobi sweeps until the maximum residual is below
+// cnTolerance; the reference aborts after (Nx-1)(Ny-1)(Nz-1) sweeps without convergence.
+static int solve_v(sr_ts *t, int c, int *iters) {
+    const FShape &f = t->fv[c];
+    int per[3] = {t->d.xPer, t->d.yPer, t->d.zPer};
+    Bulk B{};
+    for (int k = 0; k < 3; k++) {
+        B.lo[k] = 0;
+        B.hi[k] = f.n[k] - 1;
+        if (f.s[k] && !per[k]) {
+            B.lo[k] = 1;
+            B.hi[k] = f.n[k] - 2;
+        }
+    }
+    JacobiCoef J{};
+    J.hx = t->h[0]; J.hy = t->h[1]; J.hz = t->h[2];
+    J.hx2hy2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0);
+    J.hy2hz2 = std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
+    J.hz2hx2 = std::pow(J.hz, 2.0) * std::pow(J.hx, 2.0);
+    J.hx2hy2hz2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
+    J.dt = t->d.dt;
+    J.Re = t->d.Re;
+    J.inverseRe = 1.0 / t->d.Re;
+    long maxIterations = (long)(t->N[0] - 1) * (t->N[1] - 1) * (t->N[2] - 1);
+    long iterCount = 0;
+    int sweeps = 0;
+    while (true) {
+        ts_jacobi(f, t->M, B, t->V[c], t->H[c], t->G[c], J, t->s);
+        std::swap(t->V[c], t->G[c]);
+        impose_vbc(t, c);
+        ts_clear_u64(t->dmax, t->s);
+        ts_jacobi_res(f, t->M, B, t->V[c], t->H[c], J, t->dmax, t->s);
+        HIPCHK(hipMemcpyAsync(t->hmax, t->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, t->s));
+        HIPCHK(hipStreamSynchronize(t->s));
+        HIPCHK(hipGetLastError());
+        double maxError;
+        unsigned long long bits = *t->hmax;
+        std::memcpy(&maxError, &bits, sizeof(double));
+        sweeps += 1;
+        if (maxError < t->d.cnTolerance) break;
+        iterCount += 1;
+        if (iterCount > maxIterations)
+            return set_err(SR_ERR_NOT_CONVERGED, "Jacobi viscous solve did not converge within the iteration limit");
+    }
+    t->jac[c] = sweeps;
+    if (iters) *iters = sweeps;
+    return SR_OK;
+}
+
+extern "C" int sr_ts_solve_v(sr_ts *t, int component, int *iters) {
+    if (!t || component < 0 || component > 2) return set_err(SR_ERR_ARG, "bad component");
+    return solve_v(t, component, iters);
+}
+
+extern "C" int sr_ts_divergence(sr_ts *t) {
+    if (!t) return set_err(SR_ERR_ARG, "null argument");
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    ts_divergence(t->fp, t->fv, t->M, Vc, t->divV, t->h, 1.0, 0, t->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_energy(sr_ts *t, double *energy) {
+    if (!t || !energy) return set_err(SR_ERR_ARG, "null argument");
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    ts_energy(t->fp, t->fv, t->M, Vc, t->h, t->part, t->dred, t->s);
+    HIPCHK(hipMemcpyAsync(t->hred, t->dred, sizeof(double) * 2, hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
+    HIPCHK(hipGetLastError());
+    *energy = t->hred[1] / t->hred[0];
+    return SR_OK;
+}
+
+extern "C" int sr_ts_max_div(sr_ts *t, double *maxdiv) {
+    if (!t || !maxdiv) return set_err(SR_ERR_ARG, "null argument");
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    ts_divergence(t->fp, t->fv, t->M, Vc, t->divV, t->h, 1.0, 0, t->s);
+    ts_max(t->fp, t->divV, t->part, t->dred, t->s);
+    HIPCHK(hipMemcpyAsync(t->hred, t->dred, sizeof(double), hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
+    HIPCHK(hipGetLastError());
+    *maxdiv = t->hred[0];
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// the step
+// ------------------------------------------------------------------------------------------------
+extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
+    if (!t) return set_err(SR_ERR_ARG, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    const double dt = t->d.dt;
+    build_rhs(t);
+    for (int c = 0; c < 3; c++) RETCHK(solve_v(t, c, nullptr));
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    // mgRHS = div V / dt (hydro_d3.cc:336-340); pads stay 0
+    ts_divergence(t->fp, t->fv, t->M, Vc, t->mgRHS, t->h, 1.0 / dt, 1, t->s);
+    HIPCHK(hipGetLastError());
+    sr_mg_stats ms;
+    RETCHK(sr_mg_solve(t->mg, t->Pp, t->mgRHS, 1, &ms));
+    t->mg_cycles = ms.cycles;
+    ts_ghost_wave(t->fp, t->Pp, t->sync_p, t->s);
+    ts_axpy_full(t->P, t->Pp, fsize(t->fp), t->s);
+    for (int c = 0; c < 3; c++) ts_project(c, t->fv[c], t->fp, t->M, t->Pp, t->V[c], t->h[c], dt, t->s);
+    for (int c = 0; c < 3; c++) impose_vbc(t, c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        for (int c = 0; c < 3; c++) st->jacobi_iters[c] = t->jac[c];
+        st->mg_cycles = t->mg_cycles;
+        st->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    return SR_OK;
+}

@@ -1,0 +1,72 @@
+// Kernel-side types and launchers of the hydro time step (ts_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sr {
+
+// core sizes of a padded field and its staggering per direction (1 node, 0 face)
+struct FShape {
+    int n[3];
+    int s[3];
+};
+
+// device metric arrays: m[dim][0 node / 1 face][quantity][index + 1];
+// quantity 0 coordinate, 1 xi_x, 2 xi_xx, 3 xi_x^2
+struct Metrics {
+    const double *m[3][2][4];
+};
+
+// inclusive index box (bulk of a field, field.cc:206-219)
+struct Bulk {
+    int lo[3], hi[3];
+};
+
+// scalars of the Jacobi viscous solve (hydro_d3.cc:375-410)
+struct JacobiCoef {
+    double hx, hy, hz;
+    double hx2hy2, hy2hz2, hz2hx2, hx2hy2hz2;
+    double dt, Re, inverseRe;
+};
+
+// one ghost / wall assignment F(dst) = a F(src) + b
+//   kind 0: the plane index dst[0] <- src[0] along `dim`; the other two directions (in order)
+//           run over lo[0]..hi[0] and lo[1]..hi[1]
+//   kind 1: the z line at (dst[0], dst[1]) <- (src[0], src[1]), z over lo[0]..hi[0]
+struct GhostOp {
+    int kind, dim;
+    int dst[2], src[2];
+    int lo[2], hi[2];
+    double a, b;
+    long count;
+};
+
+// operations with no read-after-write dependence between them, run by one launch
+constexpr int kMaxGhostOps = 8;
+struct GhostWave {
+    int nops;
+    GhostOp ops[kMaxGhostOps];
+};
+
+void ts_deriv(const FShape &f, const Metrics &M, const double *F, double *out, int order, int d, double h,
+              hipStream_t s);
+void ts_build_rhs(int c, const FShape *fv, const FShape &fp, const Metrics &M, const double *const *V,
+                  const double *P, double *H, const double *h, double inverseRe, double dt, hipStream_t s);
+void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
+               const JacobiCoef &J, hipStream_t s);
+void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R,
+                   const JacobiCoef &J, unsigned long long *out, hipStream_t s);
+void ts_clear_u64(unsigned long long *p, hipStream_t s);
+void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s);
+void ts_divergence(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, double *out,
+                   const double *h, double scale, int scaled, hipStream_t s);
+void ts_project(int c, const FShape &fc, const FShape &fp, const Metrics &M, const double *Pp, double *V,
+                double hc, double dt, hipStream_t s);
+void ts_axpy_full(double *y, const double *x, long n, hipStream_t s);
+int ts_energy_parts(const FShape &fp);
+void ts_energy(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, const double *h,
+               double *part, double *out, hipStream_t s);
+void ts_max(const FShape &fp, const double *F, double *part, double *out, hipStream_t s);
+void ts_tgv(const FShape &f, const Metrics &M, int c, double Lx, double Ly, double Lz, double *V, hipStream_t s);
+void ts_fill(double *p, long n, double v, hipStream_t s);
+
+}  // namespace sr

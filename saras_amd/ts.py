@@ -1,0 +1,121 @@
+"""Python mirror of hydro_d3 (src/solvers/hydro/hydro_d3.cc) over the C ABI of include/saras_ts.h.
+
+Every field stays in HBM for the life of the object; `download` copies one padded field (the
+reference layout: C order, k fastest, index -1 first) to the host.  Method names follow the
+reference: timeAdvance, solveVx/Vy/Vz, imposeUBCs/VBCs/WBCs, and the tseries diagnostics."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as _L
+
+VX, VY, VZ, P, PP, MGRHS = _L.SR_TS_VX, _L.SR_TS_VY, _L.SR_TS_VZ, _L.SR_TS_P, _L.SR_TS_PP, _L.SR_TS_MGRHS
+HX, HY, HZ, DIV = _L.SR_TS_HX, _L.SR_TS_HY, _L.SR_TS_HZ, _L.SR_TS_DIV
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ts_desc(ind=(7, 7, 7), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1.0, 1.0, 1.0), Re=1600.0,
+            dt=1e-3, cn_tol=1e-6, depth=None, vc_count=5, pre=4, post=4, res_type=1, mg_tol=1e-6, prob_type=2):
+    """The parser fields of a hydro run (input/parameters.yaml keys; defaults: the Taylor-Green case)."""
+    d = _L.SrTsDesc()
+    d.xInd, d.yInd, d.zInd = ind
+    d.Lx, d.Ly, d.Lz = L
+    d.xGrid, d.yGrid, d.zGrid = (1 if c == "U" else 2 for c in mesh)
+    d.betaX, d.betaY, d.betaZ = beta
+    d.xPer, d.yPer, d.zPer = (int(c == "P") for c in domain)
+    d.probType = prob_type
+    d.Re, d.dt, d.cnTolerance = Re, dt, cn_tol
+    d.vcDepth = depth if depth is not None else min(ind) - 1
+    d.vcCount, d.preSmooth, d.postSmooth, d.resType, d.gsSmooth = vc_count, pre, post, res_type, 1
+    d.mgTolerance = mg_tol
+    return d
+
+
+class Hydro:
+    """hydro_d3 on the device: construct, set an initial condition, then timeAdvance()."""
+
+    def __init__(self, desc):
+        self.L = _L.load()
+        self.desc = desc
+        h = C.c_void_p()
+        _L.check(self.L.sr_ts_create(C.byref(desc), C.byref(h)))
+        self.h = h
+        self.stats = _L.SrTsStats()
+
+    def close(self):
+        if self.h:
+            self.L.sr_ts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- fields -----------------------------------------------------------------------------
+    def shape(self, which):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _L.check(self.L.sr_ts_field_dims(self.h, which, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value + 2, b.value + 2, c.value + 2
+
+    def download(self, which):
+        out = np.empty(self.shape(which))
+        _L.check(self.L.sr_ts_download(self.h, which, _ptr(out)))
+        return out
+
+    def upload(self, which, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape(which):
+            raise ValueError(f"field {which}: shape {a.shape} != {self.shape(which)}")
+        _L.check(self.L.sr_ts_upload(self.h, which, _ptr(a)))
+
+    # -- the step and its pieces ------------------------------------------------------------
+    def initTGV(self):
+        _L.check(self.L.sr_ts_init_tgv(self.h))
+
+    def timeAdvance(self):
+        _L.check(self.L.sr_ts_time_advance(self.h, C.byref(self.stats)))
+        return self.stats
+
+    def deriv(self, which, order, dim):
+        out = np.empty(self.shape(which))
+        _L.check(self.L.sr_ts_deriv(self.h, which, order, dim, _ptr(out)))
+        return out
+
+    def buildRHS(self):
+        _L.check(self.L.sr_ts_build_rhs(self.h))
+
+    def solveV(self, component):
+        it = C.c_int()
+        _L.check(self.L.sr_ts_solve_v(self.h, component, C.byref(it)))
+        return it.value
+
+    def solveVx(self):
+        return self.solveV(0)
+
+    def solveVy(self):
+        return self.solveV(1)
+
+    def solveVz(self):
+        return self.solveV(2)
+
+    def imposeVBC(self, component):
+        _L.check(self.L.sr_ts_impose_vbc(self.h, component))
+
+    def divergence(self):
+        _L.check(self.L.sr_ts_divergence(self.h))
+        return self.download(DIV)
+
+    def energy(self):
+        e = C.c_double()
+        _L.check(self.L.sr_ts_energy(self.h, C.byref(e)))
+        return e.value
+
+    def maxDivergence(self):
+        m = C.c_double()
+        _L.check(self.L.sr_ts_max_div(self.h, C.byref(m)))
+        return m.value

@@ -57,3 +57,18 @@ def test_create_rejects_bad_descriptor_without_device():
     d.xInd = 0
     assert lib.sr_mg_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
     assert lib.sr_grid_axis(2, 1.0, 1, 1.0, 0, None, None, None, None) == _lib.SR_ERR_ARG
+
+
+def test_ts_create_rejects_bad_descriptor_without_device():
+    from saras_amd import _lib, ts
+    lib = _lib.load()
+    h = C.c_void_p()
+    d = ts.ts_desc(ind=(5, 5, 5))
+    d.xInd = 1
+    assert lib.sr_ts_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    d = ts.ts_desc(ind=(5, 5, 5), Re=-1.0)
+    assert lib.sr_ts_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    assert b"Re" in lib.sr_last_error()
+    d = ts.ts_desc(ind=(5, 5, 5), L=(0.0, 1.0, 1.0))
+    assert lib.sr_ts_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    assert h.value is None

@@ -1,0 +1,180 @@
+"""Parity of the gfx950 hydro time step (include/saras_ts.h) with the CPU oracle's time step.
+
+Bar: every field after every piece of the step within 1e-12 relative (max-norm over the whole
+padded array, relative to the magnitude of the expected array); ghost and wall operations, which
+only copy or negate, bit for bit; equal Jacobi sweep and V-cycle counts; the kinetic-energy trace
+within 1e-12 relative.  The oracle uses the device's red-black smoother inside its multigrid.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from helpers import rel_close
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+TWO_PI = 2 * np.pi
+
+CASES = {
+    # Taylor-Green in the periodic box (the 128^3 benchmark case, smaller)
+    "tgv_ppp_33": dict(ind=(5, 5, 5), L=(TWO_PI,) * 3, domain="PPP", mesh="UUU", Re=1600.0, dt=1e-3),
+    # walls in z on a tanh grid, periodic sides, lower Re so that Jacobi iterates
+    "ppn_tanh_walls": dict(ind=(4, 4, 5), L=(1.0, 1.0, 1.0), domain="PPN", mesh="UUD", Re=50.0, dt=2e-3),
+    # closed box with the moving lid of probType 1
+    "nnn_lid": dict(ind=(4, 4, 4), L=(1.0, 1.0, 1.0), domain="NNN", mesh="UUU", Re=100.0, dt=1e-3, prob_type=1),
+    # mixed sizes, x walls only, stretched in x and y
+    "npp_stretched": dict(ind=(4, 5, 4), L=(1.0, 2.0, 1.0), domain="NPP", mesh="DDU", beta=(1.1, 0.7, 1.0),
+                          Re=200.0, dt=1e-3),
+}
+
+GPU_FIELDS = None
+
+
+def _pair(case):
+    from saras_amd import ts
+    kw = dict(CASES[case])
+    p = po.ts_params(**kw)
+    o = po.OracleTS(p)
+    d = ts.ts_desc(**kw)
+    g = ts.Hydro(d)
+    return o, g
+
+
+def _load_same(o, g, rng):
+    """Random smooth-ish fields into both (V, P), then the velocity BCs on both."""
+    from saras_amd import ts
+    for c in range(3):
+        a = o.array(c)
+        a[...] = rng.standard_normal(a.shape) * 0.1 + np.sin(np.arange(a.size).reshape(a.shape) * 0.01)
+        g.upload(c, a)
+    P = o.array(po.PRES)
+    P[...] = rng.standard_normal(P.shape)
+    g.upload(ts.P, P)
+    for c in range(3):
+        o.impose_vbc(c)
+        g.imposeVBC(c)
+
+
+def _assert_field(g, o, which_g, which_o, rtol=RTOL, exact=False, what=""):
+    a = g.download(which_g)
+    b = o.array(which_o)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    if exact:
+        assert np.array_equal(a, b), (what, float(np.max(np.abs(a - b))))
+    else:
+        ok, err, s = rel_close(a, b, rtol)
+        assert ok, (what, err, s)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_initial_condition_and_bcs(case):
+    from saras_amd import ts
+    o, g = _pair(case)
+    o.init_tgv()
+    g.initTGV()
+    for c in range(3):
+        _assert_field(g, o, c, c, rtol=1e-15, what=f"V{c}")
+    _assert_field(g, o, ts.P, po.PRES, exact=True, what="P")
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_velocity_bcs_bitwise(case):
+    """syncData + walls on random data: copies and 2v - F only, so bit for bit."""
+    o, g = _pair(case)
+    rng = np.random.default_rng(11)
+    for c in range(3):
+        a = o.array(c)
+        a[...] = rng.standard_normal(a.shape)
+        g.upload(c, a)
+        o.impose_vbc(c)
+        g.imposeVBC(c)
+        _assert_field(g, o, c, c, exact=True, what=f"V{c}")
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", ["tgv_ppp_33", "npp_stretched"])
+def test_derivatives(case):
+    from saras_amd import ts
+    o, g = _pair(case)
+    rng = np.random.default_rng(5)
+    _load_same(o, g, rng)
+    for which_g, which_o in ((ts.VX, po.VX), (ts.VY, po.VY), (ts.VZ, po.VZ), (ts.P, po.PRES)):
+        for order in (1, 2):
+            for dim in range(3):
+                a = g.deriv(which_g, order, dim)
+                b = o.deriv(which_o, order, dim)
+                ok, err, s = rel_close(a, b, RTOL)
+                assert ok, (which_g, order, dim, err, s)
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_build_rhs(case):
+    from saras_amd import ts
+    o, g = _pair(case)
+    _load_same(o, g, np.random.default_rng(7))
+    o.build_rhs()
+    g.buildRHS()
+    for c, (wg, wo) in enumerate(((ts.HX, po.HX), (ts.HY, po.HY), (ts.HZ, po.HZ))):
+        _assert_field(g, o, wg, wo, what=f"H{c}")
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_jacobi_viscous_solve(case):
+    o, g = _pair(case)
+    o.init_tgv()
+    g.initTGV()
+    o.build_rhs()
+    g.buildRHS()
+    for c in range(3):
+        it_o = o.solve_v(c)
+        it_g = g.solveV(c)
+        assert it_o == it_g, (c, it_o, it_g)
+        _assert_field(g, o, c, c, what=f"V{c}")
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_divergence_energy(case):
+    from saras_amd import ts
+    o, g = _pair(case)
+    _load_same(o, g, np.random.default_rng(9))
+    o.divergence()
+    g.divergence()
+    _assert_field(g, o, ts.DIV, po.DIV, what="div")
+    e_o, e_g = o.energy(), g.energy()
+    assert abs(e_o - e_g) <= 1e-13 * abs(e_o), (e_o, e_g)
+    m_o, m_g = o.max_div(), g.maxDivergence()
+    assert abs(m_o - m_g) <= 1e-12 * max(abs(m_o), 1e-300), (m_o, m_g)
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_time_steps(case):
+    """Whole steps from the Taylor-Green start: fields, counts and the energy trace."""
+    from saras_amd import ts
+    o, g = _pair(case)
+    o.init_tgv()
+    g.initTGV()
+    for step in range(4):
+        o.time_advance()
+        st = g.timeAdvance()
+        assert [st.jacobi_iters[c] for c in range(3)] == [o.jacobi_iters(c) for c in range(3)], step
+        assert st.mg_cycles == o.mg_cycles(), (step, st.mg_cycles, o.mg_cycles())
+        for c in range(3):
+            _assert_field(g, o, c, c, what=f"step {step} V{c}")
+        _assert_field(g, o, ts.P, po.PRES, what=f"step {step} P")
+        _assert_field(g, o, ts.PP, po.PCORR, rtol=1e-11, what=f"step {step} Pp")
+        e_o, e_g = o.energy(), g.energy()
+        assert abs(e_o - e_g) <= RTOL * abs(e_o), (step, e_o, e_g)
+    g.close()
+    o.close()
