@@ -91,6 +91,13 @@ int sr_mg_destroy(sr_mg *mg);
  * Synchronous with respect to p at return.                                                      */
 int sr_mg_solve(sr_mg *mg, double *p, const double *rhs, int on_device, sr_mg_stats *stats);
 
+/* Relative residual of a guess under the solve's own stop rule: computeError
+ * (poisson_d3.cc:391-459) evaluated for an arbitrary initial guess p and right-hand side rhs
+ * (device pointers, padded layout), loaded exactly as sr_mg_solve loads them.  With r != NULL,
+ * the residual field f - L p is written there (padded layout).  Used to judge an initial guess
+ * before a solve; the reference evaluates computeError only inside mgSolve.                     */
+int sr_mg_residual(sr_mg *mg, const double *p, const double *rhs, double *r, double *rel);
+
 /* Residual after each cycle of the last solve (the values the reference prints with
  * printResidual, poisson.cc:135-142).  Returns the count written (<= cap) or -1.                */
 int sr_mg_history(const sr_mg *mg, double *out, int cap);

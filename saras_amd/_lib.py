@@ -60,7 +60,16 @@ class SrTsStats(C.Structure):
     _fields_ = [("jacobi_iters", C.c_int * 3), ("mg_cycles", C.c_int), ("step_ms", C.c_double)]
 
 
-# (name, restype, argtypes) of every entry point declared in include/saras_mg.h and saras_ts.h
+SR_GUESS_LAYERS, SR_GUESS_IN_CH = 12, 8
+_fp = C.POINTER(C.c_float)
+
+
+class SrGuessDesc(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("seed", C.c_ulonglong), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+# (name, restype, argtypes) of every entry point declared in include/*.h
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
 SIGNATURES = [
@@ -68,6 +77,7 @@ SIGNATURES = [
     ("sr_mg_destroy", C.c_int, [_vp]),
     ("sr_mg_solve", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SrMgStats)]),
     ("sr_mg_history", C.c_int, [_vp, _dp, C.c_int]),
+    ("sr_mg_residual", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     ("sr_mg_set_level", C.c_int, [_vp, C.c_int]),
     ("sr_mg_get_level", C.c_int, [_vp, _ip]),
     ("sr_mg_set_zero_bc", C.c_int, [_vp, C.c_int]),
@@ -105,6 +115,16 @@ SIGNATURES = [
     ("sr_ts_divergence", C.c_int, [_vp]),
     ("sr_ts_energy", C.c_int, [_vp, _dp]),
     ("sr_ts_max_div", C.c_int, [_vp, _dp]),
+    ("sr_guess_create", C.c_int, [C.POINTER(SrGuessDesc), C.POINTER(C.c_void_p)]),
+    ("sr_guess_destroy", C.c_int, [_vp]),
+    ("sr_guess_layer_info", C.c_int, [C.c_int, _ip, _ip, _ip]),
+    ("sr_guess_set_layer", C.c_int, [_vp, C.c_int, _fp, _fp]),
+    ("sr_guess_get_layer", C.c_int, [_vp, C.c_int, _fp, _fp]),
+    ("sr_guess_get_grad", C.c_int, [_vp, C.c_int, _fp, _fp]),
+    ("sr_guess_forward", C.c_int, [_vp, _fp, _fp]),
+    ("sr_guess_backward", C.c_int, [_vp, _fp]),
+    ("sr_guess_adam", C.c_int, [_vp]),
+    ("sr_guess_tensor", C.c_int, [_vp, C.c_int, _fp, C.POINTER(C.c_long)]),
 ]
 
 _lib = None

@@ -245,6 +245,26 @@ static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &s
     return SR_OK;
 }
 
+// computeError (poisson_d3.cc:391-459) of an arbitrary guess: the right-hand side and the guess
+// are loaded as sr_mg_solve loads them (projection or mean removal included), the guess's ghosts
+// filled, and the stop rule's relative residual returned; optionally f - L p in padded layout.
+extern "C" int sr_mg_residual(sr_mg *m, const double *p, const double *rhs, double *r, double *rel) {
+    if (!m || !p || !rhs) return set_err(SR_ERR_ARG, "null argument");
+    Level &V = m->lv[0];
+    load_level0(m, rhs, p);
+    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), m->stream);
+    norm_level0(m);
+    HIPCHK(hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream));
+    if (r) {
+        residual_level(m, 0);
+        launch_unpack<double, double>(V.L, V.t, r, m->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (rel) *rel = error_from(m, m->hres, norm_order(m));
+    return SR_OK;
+}
+
 extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device, sr_mg_stats *st) {
     if (!m || !p || !rhs) return set_err(SR_ERR_ARG, "null argument");
     auto t0 = std::chrono::steady_clock::now();

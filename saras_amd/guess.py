@@ -1,0 +1,92 @@
+"""ctypes wrapper of the learned initial guess network (include/saras_guess.h).
+
+The network runs and trains on the GPU only; this module moves host arrays in and out for
+tests and drivers.  Layer order and tensor ids follow include/saras_guess.h."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as _L
+
+LAYER_NAMES = ["e0a", "e0b", "e1a", "e1b", "e2a", "e2b", "b_a", "b_b", "d2", "d1", "d0a", "out"]
+TENSOR_NAMES = ["x0", "a0", "s0", "x1", "a1", "s1", "x2", "a2", "s2", "x3", "a3", "bb", "u2", "g2", "u1", "g1",
+                "u0", "g0"]
+IN_CH = _L.SR_GUESS_IN_CH
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def layer_info(layer):
+    ci, co, lv = C.c_int(), C.c_int(), C.c_int()
+    _L.check(_L.load().sr_guess_layer_info(layer, C.byref(ci), C.byref(co), C.byref(lv)))
+    return ci.value, co.value, lv.value
+
+
+class GuessNet:
+    def __init__(self, n, seed=0, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.L = _L.load()
+        d = _L.SrGuessDesc()
+        d.n[0], d.n[1], d.n[2] = n
+        d.seed, d.lr, d.beta1, d.beta2, d.eps = seed, lr, beta1, beta2, eps
+        h = C.c_void_p()
+        _L.check(self.L.sr_guess_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.n = tuple(n)
+
+    def close(self):
+        if self.h:
+            self.L.sr_guess_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_layer(self, layer, W, b):
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        ci, co, _ = layer_info(layer)
+        assert W.shape == (27, co, ci) and b.shape == (co,)
+        _L.check(self.L.sr_guess_set_layer(self.h, layer, _fptr(W), _fptr(b)))
+
+    def get_layer(self, layer):
+        ci, co, _ = layer_info(layer)
+        W = np.empty((27, co, ci), np.float32)
+        b = np.empty(co, np.float32)
+        _L.check(self.L.sr_guess_get_layer(self.h, layer, _fptr(W), _fptr(b)))
+        return W, b
+
+    def get_grad(self, layer):
+        ci, co, _ = layer_info(layer)
+        W = np.empty((27, co, ci), np.float32)
+        b = np.empty(co, np.float32)
+        _L.check(self.L.sr_guess_get_grad(self.h, layer, _fptr(W), _fptr(b)))
+        return W, b
+
+    def forward(self, x0):
+        x0 = np.ascontiguousarray(x0, dtype=np.float32)
+        assert x0.shape == self.n + (IN_CH,)
+        y = np.empty(self.n, np.float32)
+        _L.check(self.L.sr_guess_forward(self.h, _fptr(x0), _fptr(y)))
+        return y
+
+    def backward(self, dy):
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        assert dy.shape == self.n
+        _L.check(self.L.sr_guess_backward(self.h, _fptr(dy)))
+
+    def adam(self):
+        _L.check(self.L.sr_guess_adam(self.h))
+
+    def tensor(self, which):
+        if isinstance(which, str):
+            which = TENSOR_NAMES.index(which)
+        cnt = C.c_long()
+        _L.check(self.L.sr_guess_tensor(self.h, which, None, C.byref(cnt)))
+        out = np.empty(cnt.value, np.float32)
+        _L.check(self.L.sr_guess_tensor(self.h, which, _fptr(out), C.byref(cnt)))
+        return out

@@ -160,6 +160,13 @@ class Multigrid:
             _L.check(code)
         return st
 
+    def residual_device(self, p_ptr, rhs_ptr, r_ptr=None):
+        """Relative residual (the stop rule's norm) of a device guess; optionally f - L p."""
+        rel = C.c_double()
+        _L.check(self.lib.sr_mg_residual(self.h, C.c_void_p(p_ptr), C.c_void_p(rhs_ptr),
+                                         C.c_void_p(r_ptr) if r_ptr else None, C.byref(rel)))
+        return rel.value
+
     def history(self, cap=1024):
         out = np.zeros(cap)
         n = self.lib.sr_mg_history(self.h, _dptr(out), cap)

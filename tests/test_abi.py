@@ -72,3 +72,18 @@ def test_ts_create_rejects_bad_descriptor_without_device():
     d = ts.ts_desc(ind=(5, 5, 5), L=(0.0, 1.0, 1.0))
     assert lib.sr_ts_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
     assert h.value is None
+
+
+def test_guess_create_rejects_bad_grid_without_device():
+    from saras_amd import _lib
+    lib = _lib.load()
+    d = _lib.SrGuessDesc()
+    d.n[0], d.n[1], d.n[2] = 17, 17, 12
+    h = C.c_void_p()
+    assert lib.sr_guess_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    assert b"2^k + 1" in lib.sr_last_error()
+    d.n[2] = 5
+    assert lib.sr_guess_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    ci, co, lv = C.c_int(), C.c_int(), C.c_int()
+    assert lib.sr_guess_layer_info(4, C.byref(ci), C.byref(co), C.byref(lv)) == 0
+    assert (ci.value, co.value, lv.value) == (32, 64, 2)

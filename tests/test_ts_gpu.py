@@ -28,8 +28,6 @@ CASES = {
                           Re=200.0, dt=1e-3),
 }
 
-GPU_FIELDS = None
-
 
 def _pair(case):
     from saras_amd import ts
@@ -56,15 +54,21 @@ def _load_same(o, g, rng):
         g.imposeVBC(c)
 
 
-def _assert_field(g, o, which_g, which_o, rtol=RTOL, exact=False, what=""):
+def _assert_field(g, o, which_g, which_o, rtol=RTOL, exact=False, what="", scale=None):
     a = g.download(which_g)
     b = o.array(which_o)
     assert a.shape == b.shape, (what, a.shape, b.shape)
     if exact:
         assert np.array_equal(a, b), (what, float(np.max(np.abs(a - b))))
     else:
-        ok, err, s = rel_close(a, b, rtol)
+        ok, err, s = rel_close(a, b, rtol, scale=scale)
         assert ok, (what, err, s)
+
+
+def _velocity_scale(o):
+    """Magnitude of the velocity field: errors of a component that is small itself (w of
+    Taylor-Green, zero at t = 0) are measured against the flow, not against that component."""
+    return max(float(np.max(np.abs(o.array(c)))) for c in range(3))
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -150,8 +154,10 @@ def test_divergence_energy(case):
     o.divergence()
     g.divergence()
     _assert_field(g, o, ts.DIV, po.DIV, what="div")
+    # the reference sums the energy sequentially (tseries.cc:158-169) and the device as a tree, so
+    # the two differ by the rounding of a long positive sum
     e_o, e_g = o.energy(), g.energy()
-    assert abs(e_o - e_g) <= 1e-13 * abs(e_o), (e_o, e_g)
+    assert abs(e_o - e_g) <= RTOL * abs(e_o), (e_o, e_g)
     m_o, m_g = o.max_div(), g.maxDivergence()
     assert abs(m_o - m_g) <= 1e-12 * max(abs(m_o), 1e-300), (m_o, m_g)
     g.close()
@@ -170,10 +176,18 @@ def test_time_steps(case):
         st = g.timeAdvance()
         assert [st.jacobi_iters[c] for c in range(3)] == [o.jacobi_iters(c) for c in range(3)], step
         assert st.mg_cycles == o.mg_cycles(), (step, st.mg_cycles, o.mg_cycles())
+        vs = _velocity_scale(o)
         for c in range(3):
-            _assert_field(g, o, c, c, what=f"step {step} V{c}")
-        _assert_field(g, o, ts.P, po.PRES, what=f"step {step} P")
-        _assert_field(g, o, ts.PP, po.PCORR, rtol=1e-11, what=f"step {step} Pp")
+            _assert_field(g, o, c, c, what=f"step {step} V{c}", scale=vs)
+        # With periodic and Neumann pressure conditions the pressure is fixed up to a constant,
+        # and the level of each correction drifts with the coarsest-level iterations on the
+        # singular system; compare the pressures with their means removed, on the scale of the
+        # pressure itself.
+        for wg, wo in ((ts.P, po.PRES), (ts.PP, po.PCORR)):
+            a, b = g.download(wg), o.array(wo)
+            a, b = a - a[1:-1, 1:-1, 1:-1].mean(), b - b[1:-1, 1:-1, 1:-1].mean()
+            ok, err, s = rel_close(a, b, RTOL, scale=float(np.max(np.abs(o.array(wo)))))
+            assert ok, (f"step {step} field {wg}", err, s)
         e_o, e_g = o.energy(), g.energy()
         assert abs(e_o - e_g) <= RTOL * abs(e_o), (step, e_o, e_g)
     g.close()
