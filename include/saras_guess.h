@@ -30,6 +30,7 @@ typedef struct sr_guess sr_guess;
 
 typedef struct sr_guess_desc {
     int n[3];                     /* level-0 nodes per direction, each 2^k + 1 with k >= 3     */
+    int fine[3];                  /* solver grid nodes: n (or 0) or 2n - 1 in every direction  */
     unsigned long long seed;      /* initial weights (He normal, zero biases)                   */
     float lr, beta1, beta2, eps;  /* Adam                                                        */
 } sr_guess_desc;
@@ -53,6 +54,30 @@ int sr_guess_adam(sr_guess *g);
 /* stored tensor `which` of the last forward as fp32 (0 x0, 1 a0, 2 s0, 3 x1, 4 a1, 5 s1, 6 x2,
  * 7 a2, 8 s2, 9 x3, 10 a3, 11 bb, 12 u2, 13 g2, 14 u1, 15 g1, 16 u0, 17 g0); *count = elements */
 int sr_guess_tensor(sr_guess *g, int which, float *host, long *count);
+
+/* ---- use inside a sequence of pressure solves --------------------------------------------- */
+typedef struct sr_guess_stats {
+    double rel_warm;              /* relative residual (sr_mg_residual) of the warm start p1     */
+    double rel_extrap;            /* ... of the extrapolation 2 p1 - p2                           */
+    double rel_learned;           /* ... of the extrapolation plus the network's correction      */
+    int choice;                   /* guess written to p_out: 0 warm start, 1 extrapolation, 2 learned */
+    double predict_ms;
+} sr_guess_stats;
+
+/* Initial guess for solving L p = rhs with solver m (whose grid is the handle's `fine` grid):
+ * p1, p2 = the two previous solutions (p2 may be NULL), all device pointers in the padded layout.
+ * Writes the best of {p1, 2 p1 - p2, 2 p1 - p2 + learned correction} by relative residual into
+ * p_out.  Keeps the network's forward pass for a following sr_guess_train.                      */
+int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const double *p2, const double *rhs,
+                     double *p_out, sr_guess_stats *stats);
+/* One training step on the converged solution p_star of the solve the last predict was for:
+ * target = p_star - (2 p1 - p2) restricted to the network grid; MSE loss, backward pass, Adam.
+ * *rel_loss = ||y - target||^2 / ||target||^2 (normalised units) before the step.               */
+int sr_guess_train(sr_guess *g, const double *p_star, double *rel_loss);
+/* p_out = 2 p1 - p2 (p1 when p2 is NULL): the extrapolated guess alone, for comparisons */
+int sr_guess_extrapolate(sr_guess *g, const double *p1, const double *p2, double *p_out);
+/* the correction scale kappa and the number of training steps taken */
+int sr_guess_kappa(const sr_guess *g, double *kappa, int *trained);
 
 #ifdef __cplusplus
 }

@@ -65,8 +65,13 @@ _fp = C.POINTER(C.c_float)
 
 
 class SrGuessDesc(C.Structure):
-    _fields_ = [("n", C.c_int * 3), ("seed", C.c_ulonglong), ("lr", C.c_float), ("beta1", C.c_float),
+    _fields_ = [("n", C.c_int * 3), ("fine", C.c_int * 3), ("seed", C.c_ulonglong), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class SrGuessStats(C.Structure):
+    _fields_ = [("rel_warm", C.c_double), ("rel_extrap", C.c_double), ("rel_learned", C.c_double),
+                ("choice", C.c_int), ("predict_ms", C.c_double)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/*.h
@@ -125,6 +130,11 @@ SIGNATURES = [
     ("sr_guess_backward", C.c_int, [_vp, _fp]),
     ("sr_guess_adam", C.c_int, [_vp]),
     ("sr_guess_tensor", C.c_int, [_vp, C.c_int, _fp, C.POINTER(C.c_long)]),
+    ("sr_guess_predict", C.c_int, [_vp, _vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(SrGuessStats)]),
+    ("sr_guess_train", C.c_int, [_vp, C.c_void_p, _dp]),
+    ("sr_guess_kappa", C.c_int, [_vp, _dp, _ip]),
+    ("sr_guess_extrapolate", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

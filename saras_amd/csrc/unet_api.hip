@@ -12,6 +12,7 @@
 #include "../../include/saras_guess.h"
 #include "mg_state.h"
 #include "unet_kernels.h"
+#include "unet_state.h"
 
 using namespace sr;
 
@@ -27,42 +28,6 @@ using namespace sr;
         int _r = (expr);                                                                              \
         if (_r != SR_OK) return _r;                                                                   \
     } while (0)
-
-namespace {
-
-struct LayerSpec {
-    int cin, cout, level;
-};
-// e0a e0b | e1a e1b | e2a e2b | b_a b_b | d2 | d1 | d0a out
-const LayerSpec kLayers[SR_GUESS_LAYERS] = {{8, 16, 0},  {16, 16, 0}, {16, 32, 1}, {32, 32, 1},
-                                            {32, 64, 2}, {64, 64, 2}, {64, 64, 3}, {64, 64, 3},
-                                            {64, 32, 2}, {32, 16, 1}, {16, 16, 0}, {16, 16, 0}};
-struct TensorSpec {
-    int level, C;
-};
-// x0 a0 s0 x1 a1 s1 x2 a2 s2 x3 a3 bb u2 g2 u1 g1 u0 g0
-const TensorSpec kTensors[18] = {{0, 8},  {0, 16}, {0, 16}, {1, 16}, {1, 32}, {1, 32}, {2, 32}, {2, 64}, {2, 64},
-                                 {3, 64}, {3, 64}, {3, 64}, {2, 64}, {2, 32}, {1, 32}, {1, 16}, {0, 16}, {0, 16}};
-enum { X0, A0, S0, X1, A1, S1, X2, A2, S2, X3, A3, BB, U2, G2, U1, G1, U0, G0 };
-
-}  // namespace
-
-struct sr_guess {
-    sr_guess_desc d{};
-    NetGrid lv[4]{};
-    hipStream_t s = nullptr;
-    float *W[SR_GUESS_LAYERS] = {}, *b[SR_GUESS_LAYERS] = {};
-    float *dW[SR_GUESS_LAYERS] = {}, *db[SR_GUESS_LAYERS] = {};
-    float *mW[SR_GUESS_LAYERS] = {}, *vW[SR_GUESS_LAYERS] = {}, *mb[SR_GUESS_LAYERS] = {}, *vb[SR_GUESS_LAYERS] = {};
-    bf16_t *pf[SR_GUESS_LAYERS] = {}, *pb[SR_GUESS_LAYERS] = {};
-    bf16_t *T[18] = {};
-    float *y = nullptr, *dyd = nullptr;
-    float *DS0 = nullptr, *DS1 = nullptr, *DS2 = nullptr, *tF = nullptr;
-    bf16_t *tM = nullptr, *dy16 = nullptr;
-    float *part = nullptr, *bpart = nullptr, *stage = nullptr;
-    int adam_t = 0;
-    std::vector<void *> owned;
-};
 
 static long tensor_elems(const sr_guess *g, int t) { return g->lv[kTensors[t].level].nvox() * kTensors[t].C; }
 static long wsize(int l) { return 27L * kLayers[l].cin * kLayers[l].cout; }
@@ -97,6 +62,12 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
         int m = n - 1;
         if (n < 9 || (m & (m - 1)) != 0) return set_err(SR_ERR_ARG, "network grid must have 2^k + 1 nodes, k >= 3");
     }
+    // solver grid: equal to the network grid, or twice as fine (2n - 1 nodes) in every direction
+    for (int k = 0; k < 3; k++) g->fine[k] = desc->fine[k] > 0 ? desc->fine[k] : desc->n[k];
+    g->ratio = g->fine[0] == desc->n[0] ? 1 : 2;
+    for (int k = 0; k < 3; k++)
+        if (g->fine[k] != (g->ratio == 1 ? desc->n[k] : 2 * desc->n[k] - 1))
+            return set_err(SR_ERR_ARG, "solver grid must equal the network grid or have 2n - 1 nodes in every direction");
     int n[3] = {desc->n[0], desc->n[1], desc->n[2]};
     for (int l = 0; l < 4; l++) {
         g->lv[l] = net_grid(n[0], n[1], n[2]);
@@ -136,6 +107,14 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
     RETCHK(dalloc(g, &g->part, partmax));
     RETCHK(dalloc(g, &g->bpart, 512L * 64));
     RETCHK(dalloc(g, &g->stage, n0 * SR_GUESS_IN_CH));
+    g->fine_pad = (long)(g->fine[0] + 2) * (g->fine[1] + 2) * (g->fine[2] + 2);
+    RETCHK(dalloc(g, &g->extrap, g->fine_pad));
+    RETCHK(dalloc(g, &g->rfield, g->fine_pad));
+    RETCHK(dalloc(g, &g->stage4, n0 * 4));
+    RETCHK(dalloc(g, &g->target, n0));
+    RETCHK(dalloc(g, &g->rpart, 4096L * 4));
+    RETCHK(dalloc(g, &g->dsc, 16));
+    HIPCHK(hipHostMalloc((void **)&g->hsc, sizeof(double) * 16, hipHostMallocDefault));
     // He-normal initial weights (zero biases); the unused input channels and output channels
     // start at zero, the output layer small
     uint64_t st = desc->seed;
@@ -164,6 +143,7 @@ extern "C" int sr_guess_destroy(sr_guess *g) {
     if (!g) return SR_OK;
     if (g->s) (void)hipStreamSynchronize(g->s);
     for (void *p : g->owned) (void)hipFree(p);
+    if (g->hsc) (void)hipHostFree(g->hsc);
     if (g->s) (void)hipStreamDestroy(g->s);
     delete g;
     return SR_OK;
@@ -243,7 +223,7 @@ static int wgrad(sr_guess *g, int l, const bf16_t *dY, const bf16_t *X) {
     return SR_OK;
 }
 
-int guess_forward_device(sr_guess *g) {
+int sr::guess_forward_device(sr_guess *g) {
     const NetGrid *v = g->lv;
     RETCHK(conv(g, 0, g->T[X0], g->T[A0], CONV_BF16_RELU));
     RETCHK(conv(g, 1, g->T[A0], g->T[S0], CONV_BF16_RELU));
@@ -267,7 +247,7 @@ int guess_forward_device(sr_guess *g) {
 }
 
 // gradients of sum(dyd * y); the mask of each ReLU is the stored post-activation (> 0)
-int guess_backward_device(sr_guess *g) {
+int sr::guess_backward_device(sr_guess *g) {
     const NetGrid *v = g->lv;
     hipStream_t s = g->s;
     const long n0 = v[0].nvox(), n1 = v[1].nvox(), n2 = v[2].nvox(), n3 = v[3].nvox();
@@ -315,7 +295,7 @@ int guess_backward_device(sr_guess *g) {
     return SR_OK;
 }
 
-int guess_adam_device(sr_guess *g) {
+int sr::guess_adam_device(sr_guess *g) {
     g->adam_t += 1;
     const double b1 = g->d.beta1, b2 = g->d.beta2;
     const float lr_t = (float)(g->d.lr * std::sqrt(1.0 - std::pow(b2, g->adam_t)) / (1.0 - std::pow(b1, g->adam_t)));

@@ -25,10 +25,12 @@ def layer_info(layer):
 
 
 class GuessNet:
-    def __init__(self, n, seed=0, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    def __init__(self, n, seed=0, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, fine=None):
         self.L = _L.load()
         d = _L.SrGuessDesc()
         d.n[0], d.n[1], d.n[2] = n
+        if fine is not None:
+            d.fine[0], d.fine[1], d.fine[2] = fine
         d.seed, d.lr, d.beta1, d.beta2, d.eps = seed, lr, beta1, beta2, eps
         h = C.c_void_p()
         _L.check(self.L.sr_guess_create(C.byref(d), C.byref(h)))
@@ -90,3 +92,25 @@ class GuessNet:
         out = np.empty(cnt.value, np.float32)
         _L.check(self.L.sr_guess_tensor(self.h, which, _fptr(out), C.byref(cnt)))
         return out
+
+    # -- inside a sequence of solves (device pointers, padded layout) ---------------------------
+    def predict(self, solver, p1_ptr, p2_ptr, rhs_ptr, out_ptr):
+        st = _L.SrGuessStats()
+        _L.check(self.L.sr_guess_predict(self.h, solver.h, C.c_void_p(p1_ptr),
+                                         C.c_void_p(p2_ptr) if p2_ptr else None, C.c_void_p(rhs_ptr),
+                                         C.c_void_p(out_ptr), C.byref(st)))
+        return st
+
+    def train(self, pstar_ptr):
+        loss = C.c_double()
+        _L.check(self.L.sr_guess_train(self.h, C.c_void_p(pstar_ptr), C.byref(loss)))
+        return loss.value
+
+    def kappa(self):
+        k, t = C.c_double(), C.c_int()
+        _L.check(self.L.sr_guess_kappa(self.h, C.byref(k), C.byref(t)))
+        return k.value, t.value
+
+    def extrapolate(self, p1_ptr, p2_ptr, out_ptr):
+        _L.check(self.L.sr_guess_extrapolate(self.h, C.c_void_p(p1_ptr), C.c_void_p(p2_ptr) if p2_ptr else None,
+                                             C.c_void_p(out_ptr)))

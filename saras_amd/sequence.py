@@ -1,0 +1,100 @@
+"""Sequences of pressure solves with different initial guesses, all in HBM.
+
+A time-stepping run solves L p_n = f_n once per step with f_n changing smoothly.  The variants
+below differ only in the initial guess handed to the multigrid solve and in its precision:
+
+  fp64_zero     FP64 multigrid from p = 0
+  fp64_warm     FP64 multigrid from the previous solution (what the reference's mgSolve gets:
+                hydro_d3 passes the previous pressure correction in, hydro_d3.cc:343)
+  mixed_warm    FP32 V-cycles under FP64 refinement, from the previous solution
+  mixed_extrap  mixed precision from 2 p_{n-1} - p_{n-2}
+  learned_mixed mixed precision from the learned guess (include/saras_guess.h), trained online on
+                the solves of the first `ntrain` steps, then frozen
+  learned_fp64  FP64 multigrid from the learned guess
+
+Each solve runs to the same stop rule (relative residual 1e-10 on the projected right-hand side);
+the time of a step is the guess plus the solve, with the right-hand side already in HBM."""
+import copy
+import time
+
+import numpy as np
+
+from . import _lib as _L
+from . import mg
+
+VARIANTS = ("fp64_zero", "fp64_warm", "mixed_warm", "mixed_extrap", "learned_mixed", "learned_fp64")
+
+
+def net_grid_for(shape):
+    """The guess network runs at half the solver's resolution: (N - 1) / 2 + 1 nodes."""
+    return tuple((k - 2 - 1) // 2 + 1 for k in shape)
+
+
+def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None):
+    """Run `nsteps` solves of the sequence `seq` (workloads.EvolvingPoisson).  Returns a list of
+    per-step records and the network handle used (if any)."""
+    assert variant in VARIANTS, variant
+    p = copy.copy(seq.p)
+    p.precision = _L.SR_PREC_MIXED if variant.startswith("mixed") or variant == "learned_mixed" else _L.SR_PREC_FP64
+    solver = mg.Multigrid(seq.g, p)
+    shape = seq.shape
+    d_f = _L.DeviceArray(shape)
+    d_zero = _L.DeviceArray(shape)
+    d_zero.upload(np.zeros(shape))
+    ring = [_L.DeviceArray(shape) for _ in range(3)]
+    lib = _L.load()
+    need_net = variant in ("mixed_extrap", "learned_mixed", "learned_fp64")
+    if need_net and net is None:
+        from .guess import GuessNet
+        net = GuessNet(net_grid_for(shape), seed=seed, lr=lr, fine=tuple(k - 2 for k in shape))
+    recs = []
+    for n in range(nsteps):
+        d_f.upload(seq.rhs(n))
+        cur = ring[n % 3]
+        p1 = ring[(n - 1) % 3] if n >= 1 else None
+        p2 = ring[(n - 2) % 3] if n >= 2 else None
+        rec = {"step": n}
+        t0 = time.perf_counter()
+        if variant == "fp64_zero" or p1 is None:
+            _L.check(lib.sr_memcpy(cur.ptr, d_zero.ptr, cur.nbytes, 3))
+        elif variant.endswith("warm"):
+            _L.check(lib.sr_memcpy(cur.ptr, p1.ptr, cur.nbytes, 3))
+        elif variant == "mixed_extrap":
+            net.extrapolate(p1.ptr, p2.ptr if p2 else None, cur.ptr)
+        else:
+            st = net.predict(solver, p1.ptr, p2.ptr if p2 else None, d_f.ptr, cur.ptr)
+            rec.update(choice=st.choice, rel_warm=st.rel_warm, rel_extrap=st.rel_extrap,
+                       rel_learned=st.rel_learned, predict_ms=st.predict_ms)
+        t1 = time.perf_counter()
+        stats = solver.mgSolve_device(cur.ptr, d_f.ptr)
+        t2 = time.perf_counter()
+        rec.update(ms=(t2 - t0) * 1e3, guess_ms=(t1 - t0) * 1e3, solve_ms=(t2 - t1) * 1e3, cycles=stats.cycles,
+                   converged=bool(stats.converged), residual=stats.residual)
+        hist = solver.history()
+        rec["first_cycle_residual"] = float(hist[0]) if len(hist) else None
+        if variant.startswith("learned") and p1 is not None and n < ntrain:
+            rec["train_loss"] = net.train(cur.ptr)
+        recs.append(rec)
+        if on_step:
+            on_step(rec)
+    last = [r.download() for r in (ring[(nsteps - 1) % 3],)]
+    solver.close()
+    for a in ring + [d_f, d_zero]:
+        a.free()
+    return recs, net, last[0]
+
+
+def summary(recs, first):
+    """Mean ms, cycles and solve time over steps >= first."""
+    sel = [r for r in recs if r["step"] >= first]
+    out = {"steps": len(sel), "ms_per_solve": float(np.mean([r["ms"] for r in sel])),
+           "cycles_mean": float(np.mean([r["cycles"] for r in sel])),
+           "all_converged": all(r["converged"] for r in sel),
+           "max_final_residual": float(max(r["residual"] for r in sel)),
+           "guess_ms_mean": float(np.mean([r["guess_ms"] for r in sel]))}
+    if "choice" in sel[0]:
+        out["learned_chosen"] = int(sum(1 for r in sel if r.get("choice") == 2))
+        out["rel_extrap_median"] = float(np.median([r["rel_extrap"] for r in sel]))
+        out["rel_learned_median"] = float(np.median([r["rel_learned"] for r in sel]))
+        out["rel_warm_median"] = float(np.median([r["rel_warm"] for r in sel]))
+    return out

@@ -75,3 +75,39 @@ def stretched_poisson(ind=8, beta=1.0, seed=0, depth=None):
     g = mg.Grid(p)
     f = fourier_rhs(g.xStaggr, g.yStaggr, g.zStaggr, seed)
     return p, g, f
+
+
+class EvolvingPoisson:
+    """A sequence of right-hand sides on the stretched-grid workload, as a time-stepping run
+    produces them: the seeded modes of `fourier_rhs` with amplitudes a_m cos(w_m t + psi_m),
+    w_m uniform in [0.5, 2], psi_m uniform in [0, 2 pi), at t = step * dt.  The solutions form a
+    smooth sequence in t, so the previous solutions carry information about the next one."""
+
+    def __init__(self, ind=8, beta=1.0, seed=0, dt=0.05, depth=None):
+        self.p, self.g, _ = stretched_poisson(ind=ind, beta=beta, seed=seed, depth=depth)
+        self.dt = dt
+        modes = fourier_modes(seed)
+        gen = SplitMix64(seed + 7919)
+        self.omega = [0.5 + 1.5 * gen.uniform() for _ in modes]
+        self.psi = [2.0 * math.pi * gen.uniform() for _ in modes]
+        g = self.g
+        xc, yc, zc = g.xStaggr[1:-1], g.yStaggr[1:-1], g.zStaggr[1:-1]
+        self.shape = (len(g.xStaggr), len(g.yStaggr), len(g.zStaggr))
+        self.fields = np.empty((len(modes), len(xc) * len(yc) * len(zc)))
+        self.amp = []
+        for m, (a, kx, ky, kz, phx, phy) in enumerate(modes):
+            cx = np.cos(2.0 * math.pi * kx * xc + phx)
+            cy = np.cos(2.0 * math.pi * ky * yc + phy)
+            cz = np.cos(math.pi * kz * zc)
+            self.fields[m] = (cx[:, None, None] * cy[None, :, None] * cz[None, None, :]).ravel()
+            self.amp.append(a)
+
+    def coefficients(self, step):
+        t = step * self.dt
+        return np.array([a * math.cos(w * t + s) for a, w, s in zip(self.amp, self.omega, self.psi)])
+
+    def rhs(self, step):
+        f = np.zeros(self.shape)
+        f[1:-1, 1:-1, 1:-1] = (self.coefficients(step) @ self.fields).reshape(
+            tuple(k - 2 for k in self.shape))
+        return f
