@@ -56,8 +56,11 @@ struct or_ts {
     double *P, *Pp, *mgRHS, *divV;
     double *V[3], *H[3], *G[3], *guess[3], *lap[3], *tmpA, *tmpB, *inter, *deriv;
     or_mg *mg;
-    int jacobi_iters[3];
+    int jacobi_iters[4];
     int mg_cycles;
+    /* scalar_d3 */
+    double *T, *Trhs, *Tg, *Tlap;
+    double nu, kappa, Fb;
 };
 
 /* ------------------------------------------------------------------------------------------ */
@@ -107,6 +110,22 @@ or_ts *or_ts_create(const or_ts_params *p) {
     t->tmpB = (double *)calloc((size_t)nmax, sizeof(double));
     t->inter = (double *)calloc((size_t)nmax, sizeof(double));
     t->deriv = (double *)calloc((size_t)nmax, sizeof(double));
+    t->T = (double *)calloc((size_t)np, sizeof(double));
+    t->Trhs = (double *)calloc((size_t)np, sizeof(double));
+    t->Tg = (double *)calloc((size_t)np, sizeof(double));
+    t->Tlap = (double *)calloc((size_t)np, sizeof(double));
+    /* scalar.cc:69-86 and buoyantForce.cc:45-55 */
+    t->nu = 1.0 / p->Re;
+    t->kappa = 0.0;
+    t->Fb = 0.0;
+    if (p->scalar) {
+        switch (p->rbcType) {
+            case 1: t->nu = p->Pr; t->kappa = 1.0; t->Fb = p->Ra * p->Pr; break;
+            case 3: t->nu = 1.0; t->kappa = 1.0 / p->Pr; t->Fb = p->Ra; break;
+            case 4: t->nu = sqrt(p->Pr / p->Ra); t->kappa = 1.0 / sqrt(p->Pr * p->Ra); t->Fb = p->Pr; break;
+            default: t->nu = sqrt(p->Pr / p->Ra); t->kappa = 1.0 / sqrt(p->Pr * p->Ra); t->Fb = 1.0; break;
+        }
+    }
     /* pressure solver on the node grid (hydro_d3.cc:63-66) */
     or_params mp;
     memset(&mp, 0, sizeof(mp));
@@ -115,8 +134,9 @@ or_ts *or_ts_create(const or_ts_params *p) {
     mp.preSmooth = p->preSmooth; mp.postSmooth = p->postSmooth;
     mp.resType = p->resType; mp.smoother = p->smoother;
     mp.xPer = p->xPer; mp.yPer = p->yPer; mp.zPer = p->zPer;
-    mp.semantics = OR_SEM_REFERENCE;
+    mp.semantics = p->semantics;
     mp.mgTolerance = p->mgTolerance;
+    mp.relTolerance = p->relTolerance;
     mp.dXi = t->h[0]; mp.dEt = t->h[1]; mp.dZt = t->h[2];
     t->mg = or_mg_create(&mp, t->met[0][0][3], t->met[0][0][2], t->met[1][0][3], t->met[1][0][2],
                          t->met[2][0][3], t->met[2][0][2], NULL, NULL, NULL, NULL, NULL, NULL);
@@ -130,6 +150,7 @@ void or_ts_destroy(or_ts *t) {
     free(t->P); free(t->Pp); free(t->mgRHS); free(t->divV);
     for (int c = 0; c < 3; c++) { free(t->V[c]); free(t->H[c]); free(t->G[c]); free(t->guess[c]); free(t->lap[c]); }
     free(t->tmpA); free(t->tmpB); free(t->inter); free(t->deriv);
+    free(t->T); free(t->Trhs); free(t->Tg); free(t->Tlap);
     or_mg_destroy(t->mg);
     free(t);
 }
@@ -143,6 +164,8 @@ double *or_ts_array(or_ts *t, int which) {
         case 6: case 7: case 8: return t->H[which - 6];
         case 9: case 10: case 11: return t->G[which - 9];
         case 12: return t->divV;
+        case 13: return t->T;
+        case 14: return t->Trhs;
         default: return NULL;
     }
 }
@@ -450,7 +473,8 @@ int or_ts_solve_v(or_ts *t, int c) {
     double hx2hy2 = pow(hx, 2.0) * pow(hy, 2.0), hy2hz2 = pow(hy, 2.0) * pow(hz, 2.0);
     double hz2hx2 = pow(hz, 2.0) * pow(hx, 2.0);
     double hx2hy2hz2 = pow(hx, 2.0) * pow(hy, 2.0) * pow(hz, 2.0);
-    double dt = t->p.dt, Re = t->p.Re, inverseRe = 1.0 / Re;
+    double dt = t->p.dt, Re = t->p.Re, nu = t->nu;
+    double visc = t->p.scalar ? nu : 1.0 / Re;
     int per[3] = {t->p.xPer, t->p.yPer, t->p.zPer};
     int lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
@@ -467,9 +491,13 @@ int or_ts_solve_v(or_ts *t, int c) {
                 for (int k = lo[2]; k <= hi[2]; k++) {
                     long q = FI(f, i, j, k);
                     double mx = met(t, 0, f->s[0], 3, i), my = met(t, 1, f->s[1], 3, j), mz = met(t, 2, f->s[2], 3, k);
-                    Gs[q] = ((hy2hz2 * mx * (V[q + st[0]] + V[q - st[0]]) + hz2hx2 * my * (V[q + st[1]] + V[q - st[1]]) +
-                              hx2hy2 * mz * (V[q + 1] + V[q - 1])) * dt / (hx2hy2hz2 * 2.0 * Re) + R[q]) /
-                            (1.0 + dt * ((hy2hz2 * mx + hz2hx2 * my + hx2hy2 * mz)) / (Re * hx2hy2hz2));
+                    double a = hy2hz2 * mx * (V[q + st[0]] + V[q - st[0]]) + hz2hx2 * my * (V[q + st[1]] + V[q - st[1]]) +
+                               hx2hy2 * mz * (V[q + 1] + V[q - 1]);
+                    double d = hy2hz2 * mx + hz2hx2 * my + hx2hy2 * mz;
+                    if (t->p.scalar)      /* scalar_d3.cc:378-386 */
+                        Gs[q] = (a * dt * nu / (hx2hy2hz2 * 2.0) + R[q]) / (1.0 + dt * nu * (d) / hx2hy2hz2);
+                    else                  /* hydro_d3.cc:378-386 */
+                        Gs[q] = (a * dt / (hx2hy2hz2 * 2.0 * Re) + R[q]) / (1.0 + dt * (d) / (Re * hx2hy2hz2));
                 }
         memcpy(V, Gs, sizeof(double) * (size_t)fsize(f));
         or_ts_impose_vbc(t, c);
@@ -481,7 +509,7 @@ int or_ts_solve_v(or_ts *t, int c) {
                     double mx = met(t, 0, f->s[0], 3, i), my = met(t, 1, f->s[1], 3, j), mz = met(t, 2, f->s[2], 3, k);
                     Lp[q] = V[q] - (mx * (V[q + st[0]] - 2.0 * V[q] + V[q - st[0]]) / (hx * hx) +
                                     my * (V[q + st[1]] - 2.0 * V[q] + V[q - st[1]]) / (hy * hy) +
-                                    mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (hz * hz)) * 0.5 * dt * inverseRe;
+                                    mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (hz * hz)) * 0.5 * dt * visc;
                     double e = fabs(Lp[q] - R[q]);
                     if (e > maxError) maxError = e;
                 }
@@ -499,14 +527,28 @@ int or_ts_solve_v(or_ts *t, int c) {
 /* ------------------------------------------------------------------------------------------ */
 /* The first half: nseRHS = V + dt (nu-free diffusion / Re - advection - grad P), synced.       */
 void or_ts_build_rhs(or_ts *t) {
-    double inverseRe = 1.0 / t->p.Re, dt = t->p.dt;
+    /* hydro_d3 multiplies by 1/Re (hydro_d3.cc:259-262), scalar_d3 by nu (scalar_d3.cc:254-256) */
+    double visc = t->p.scalar ? t->nu : 1.0 / t->p.Re, dt = t->p.dt;
     for (int c = 0; c < 3; c++) memset(t->H[c], 0, sizeof(double) * (size_t)fsize(&t->sV[c]));
     or_ts_compute_diff(t);
     for (int c = 0; c < 3; c++) {
         long n = fsize(&t->sV[c]);
-        for (long q = 0; q < n; q++) t->H[c][q] *= inverseRe;
+        for (long q = 0; q < n; q++) t->H[c][q] *= visc;
     }
     or_ts_compute_nlin(t);
+    if (t->p.scalar) {
+        /* buoyantForce::addForcing (buoyantForce.cc:59-68): Vz += Fb (T(k) + T(k+1)) / 2 */
+        const fshape *f = &t->sV[2];
+        for (int i = 0; i < f->n[0]; i++)
+            for (int j = 0; j < f->n[1]; j++)
+                for (int k = 0; k < f->n[2]; k++) {
+                    double inter = 0.0;
+                    inter += t->T[FI(&t->sP, i, j, k)];
+                    inter += t->T[FI(&t->sP, i, j, k + 1)];
+                    inter /= 2.0;
+                    t->H[2][FI(f, i, j, k)] += t->Fb * inter;
+                }
+    }
     or_ts_gradient(t, 0);
     for (int c = 0; c < 3; c++) {
         long n = fsize(&t->sV[c]);
@@ -517,6 +559,160 @@ void or_ts_build_rhs(or_ts *t) {
         }
         sync_data(t, &t->sV[c], t->H[c]);
     }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* scalar_d3: temperature equation, its BCs and the RBC initial state                          */
+/* ------------------------------------------------------------------------------------------ */
+/* scalar::imposeTBCs (scalar.cc:240-255) with the probType 5 walls of initTBC (scalar.cc:196-228):
+   syncData, adiabatic (neumannCC) x and y walls, then the cold top (0) and the hot bottom (1) */
+void or_ts_impose_tbc(or_ts *t) {
+    const fshape *f = &t->sP;
+    double *F = t->T;
+    sync_data(t, f, F);
+    if (!t->p.xPer) { wall_bc(f, 0, F, 0, 4, 0.0); wall_bc(f, 0, F, 1, 4, 0.0); }
+    if (!t->p.yPer) { wall_bc(f, 0, F, 2, 4, 0.0); wall_bc(f, 0, F, 3, 4, 0.0); }
+    if (t->p.zPer) {
+        wall_bc(f, 1, F, 5, 0, 0.0);
+        wall_bc(f, 1, F, 4, 0, 0.0);
+    } else {
+        wall_bc(f, 0, F, 5, 2, 0.0);
+        wall_bc(f, 0, F, 4, 2, 1.0);
+    }
+}
+
+/* the temperature right-hand side of scalar_d3::timeAdvance (scalar_d3.cc:349-360):
+   tmpRHS = T + dt (kappa sum d2 T - sum_d (V_d interpolated) dT/dx_d), then syncData.  The
+   interpolation of V_d onto the nodes averages the faces on either side (field.cc:412-434). */
+void or_ts_build_trhs(or_ts *t) {
+    const fshape *p = &t->sP;
+    long np = fsize(p);
+    memset(t->Trhs, 0, sizeof(double) * (size_t)np);
+    for (int d = 0; d < 3; d++) {
+        memset(t->deriv, 0, sizeof(double) * (size_t)np);
+        d2(t, p, t->T, d, t->deriv);
+        for (int i = 0; i < p->n[0]; i++)
+            for (int j = 0; j < p->n[1]; j++)
+                for (int k = 0; k < p->n[2]; k++) t->Trhs[FI(p, i, j, k)] += t->deriv[FI(p, i, j, k)];
+    }
+    for (long q = 0; q < np; q++) t->Trhs[q] *= t->kappa;
+    for (int d = 0; d < 3; d++) {
+        const fshape *fd = &t->sV[d];
+        memset(t->deriv, 0, sizeof(double) * (size_t)np);
+        d1(t, p, t->T, d, t->deriv);
+        for (int i = 0; i < p->n[0]; i++)
+            for (int j = 0; j < p->n[1]; j++)
+                for (int k = 0; k < p->n[2]; k++) {
+                    int o[3] = {i, j, k};
+                    o[d] -= 1;
+                    double inter = 0.0;
+                    inter += t->V[d][FI(fd, i, j, k)];
+                    inter += t->V[d][FI(fd, o[0], o[1], o[2])];
+                    long q = FI(p, i, j, k);
+                    t->Trhs[q] -= inter * t->deriv[q] / 2.0;
+                }
+    }
+    for (long q = 0; q < np; q++) {
+        t->Trhs[q] *= t->p.dt;
+        t->Trhs[q] += t->T[q];
+    }
+    sync_data(t, p, t->Trhs);
+}
+
+/* scalar_d3::solveT (scalar_d3.cc:533-594).  The z metric is ztz2Colloc read at node indices, as
+   the reference does (scalar_d3.cc:541, 561). */
+int or_ts_solve_t(or_ts *t) {
+    const fshape *f = &t->sP;
+    double *T = t->T, *R = t->Trhs, *Gs = t->Tg, *Lp = t->Tlap;
+    double hx = t->h[0], hy = t->h[1], hz = t->h[2];
+    double hx2hy2 = pow(hx, 2.0) * pow(hy, 2.0), hy2hz2 = pow(hy, 2.0) * pow(hz, 2.0);
+    double hz2hx2 = pow(hz, 2.0) * pow(hx, 2.0);
+    double hx2hy2hz2 = pow(hx, 2.0) * pow(hy, 2.0) * pow(hz, 2.0);
+    double dt = t->p.dt, kappa = t->kappa;
+    int per[3] = {t->p.xPer, t->p.yPer, t->p.zPer};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = 0;
+        hi[d] = f->n[d] - 1;
+        if (!per[d]) { lo[d] = 1; hi[d] = f->n[d] - 2; }
+    }
+    long st[3] = {(long)(f->n[1] + 2) * (f->n[2] + 2), (long)(f->n[2] + 2), 1};
+    int maxIterations = (t->N[0] - 1) * (t->N[1] - 1) * (t->N[2] - 1);
+    int iterCount = 0, sweeps = 0;
+    while (1) {
+        for (int i = lo[0]; i <= hi[0]; i++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int k = lo[2]; k <= hi[2]; k++) {
+                    long q = FI(f, i, j, k);
+                    double mx = met(t, 0, 1, 3, i), my = met(t, 1, 1, 3, j), mz = met(t, 2, 0, 3, k);
+                    double a = hy2hz2 * mx * (T[q + st[0]] + T[q - st[0]]) + hz2hx2 * my * (T[q + st[1]] + T[q - st[1]]) +
+                               hx2hy2 * mz * (T[q + 1] + T[q - 1]);
+                    double d = hy2hz2 * mx + hz2hx2 * my + hx2hy2 * mz;
+                    Gs[q] = (a * dt * kappa / (hx2hy2hz2 * 2.0) + R[q]) / (1.0 + dt * kappa * (d) / hx2hy2hz2);
+                }
+        memcpy(T, Gs, sizeof(double) * (size_t)fsize(f));
+        or_ts_impose_tbc(t);
+        double maxError = 0.0;
+        for (int i = lo[0]; i <= hi[0]; i++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int k = lo[2]; k <= hi[2]; k++) {
+                    long q = FI(f, i, j, k);
+                    double mx = met(t, 0, 1, 3, i), my = met(t, 1, 1, 3, j), mz = met(t, 2, 0, 3, k);
+                    Lp[q] = T[q] - (mx * (T[q + st[0]] - 2.0 * T[q] + T[q - st[0]]) / (hx * hx) +
+                                    my * (T[q + st[1]] - 2.0 * T[q] + T[q - st[1]]) / (hy * hy) +
+                                    mz * (T[q + 1] - 2.0 * T[q] + T[q - 1]) / (hz * hz)) * 0.5 * dt * kappa;
+                    double e = fabs(Lp[q] - R[q]);
+                    if (e > maxError) maxError = e;
+                }
+        sweeps += 1;
+        if (maxError < t->p.cnTolerance) break;
+        iterCount += 1;
+        if (iterCount > maxIterations) return -1;
+    }
+    t->jacobi_iters[3] = sweeps;
+    return sweeps;
+}
+
+static unsigned long long sm_next(unsigned long long *s) {
+    *s += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = *s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* scalar_d3 constructor (scalar_d3.cc:85-107): P = 1, T = 0 (here plus a seeded perturbation on
+   the core, the convection case's initial state), V = 0, then the velocity and temperature BCs */
+void or_ts_init_rbc(or_ts *t, unsigned long long seed, double amp) {
+    long np = fsize(&t->sP);
+    for (long q = 0; q < np; q++) { t->P[q] = 1.0; t->Pp[q] = 0.0; t->T[q] = 0.0; }
+    for (int c = 0; c < 3; c++) memset(t->V[c], 0, sizeof(double) * (size_t)fsize(&t->sV[c]));
+    unsigned long long st = seed;
+    const fshape *p = &t->sP;
+    for (int i = 0; i < p->n[0]; i++)
+        for (int j = 0; j < p->n[1]; j++)
+            for (int k = 0; k < p->n[2]; k++) {
+                double u = ((double)(sm_next(&st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+                t->T[FI(p, i, j, k)] = amp * (2.0 * u - 1.0);
+            }
+    for (int c = 0; c < 3; c++) or_ts_impose_vbc(t, c);
+    or_ts_impose_tbc(t);
+}
+
+/* tseries::writeTSData(T, nu, kappa) (tseries.cc:195-246): Nu = 1 + <w T> / kappa, w averaged
+   from the two z faces of each node, volume-weighted over every core node */
+double or_ts_nusselt(const or_ts *t) {
+    double vol = 0.0, uzt = 0.0;
+    const fshape *p = &t->sP;
+    for (int i = 0; i < p->n[0]; i++)
+        for (int j = 0; j < p->n[1]; j++)
+            for (int k = 0; k < p->n[2]; k++) {
+                double dv = (t->h[0] / met(t, 0, 0, 1, i)) * (t->h[1] / met(t, 1, 0, 1, j)) * (t->h[2] / met(t, 2, 0, 1, k));
+                vol += dv;
+                uzt += ((t->V[2][FI(&t->sV[2], i, j, k)] + t->V[2][FI(&t->sV[2], i, j, k - 1)]) / 2.0) *
+                       t->T[FI(p, i, j, k)] * dv;
+            }
+    return 1.0 + (uzt / vol) / t->kappa;
 }
 
 int or_ts_time_advance(or_ts *t) {
@@ -538,6 +734,11 @@ int or_ts_time_advance(or_ts *t) {
             t->V[c][q] -= t->G[c][q];
         }
     }
+    if (t->p.scalar) {
+        or_ts_build_trhs(t);
+        if (or_ts_solve_t(t) < 0) return -1;
+    }
     for (int c = 0; c < 3; c++) or_ts_impose_vbc(t, c);
+    if (t->p.scalar) or_ts_impose_tbc(t);
     return 0;
 }

@@ -117,13 +117,19 @@ typedef struct or_ts_params {
     double Re, dt, cnTolerance;
     int vcDepth, vcCount, preSmooth, postSmooth, resType, smoother;
     double mgTolerance;
+    int scalar;            /* 1: scalar_d3 (temperature with buoyancy forcing, forceType 3)   */
+    int rbcType;           /* non-dimensionalisation of scalar.cc:69-86                         */
+    double Ra, Pr;
+    int semantics;         /* multigrid stop rule of the pressure solve (OR_SEM_*)              */
+    double relTolerance;
 } or_ts_params;
 
 typedef struct or_ts or_ts;
 
 or_ts *or_ts_create(const or_ts_params *p);
 void   or_ts_destroy(or_ts *t);
-/* which: 0..2 Vx Vy Vz, 3 P, 4 Pp, 5 mgRHS, 6..8 nseRHS, 9..11 pressure gradient, 12 divergence */
+/* which: 0..2 Vx Vy Vz, 3 P, 4 Pp, 5 mgRHS, 6..8 nseRHS, 9..11 pressure gradient, 12 divergence,
+   13 T, 14 tmpRHS */
 double *or_ts_array(or_ts *t, int which);
 void   or_ts_shape(const or_ts *t, int which, int *nx, int *ny, int *nz);
 /* dim 0..2, node 1 = staggered / 0 = collocated, quantity 0 x, 1 xi_x, 2 xi_xx, 3 xi_x^2 */
@@ -140,7 +146,14 @@ double or_ts_max_div(or_ts *t);
 int    or_ts_solve_v(or_ts *t, int component);
 void   or_ts_build_rhs(or_ts *t);
 int    or_ts_time_advance(or_ts *t);
-int    or_ts_jacobi_iters(const or_ts *t, int component);
+int    or_ts_jacobi_iters(const or_ts *t, int component);   /* component 3: temperature */
+/* scalar_d3: V = 0, P = 1, T = amp (2u - 1) on the core (u uniform from SplitMix64(seed) in C
+   order), then the velocity and temperature BCs */
+void   or_ts_init_rbc(or_ts *t, unsigned long long seed, double amp);
+void   or_ts_impose_tbc(or_ts *t);
+int    or_ts_solve_t(or_ts *t);
+void   or_ts_build_trhs(or_ts *t);
+double or_ts_nusselt(const or_ts *t);
 int    or_ts_mg_cycles(const or_ts *t);
 
 #ifdef __cplusplus

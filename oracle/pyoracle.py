@@ -232,10 +232,12 @@ class TsParams(C.Structure):
         ("vcDepth", C.c_int), ("vcCount", C.c_int), ("preSmooth", C.c_int), ("postSmooth", C.c_int),
         ("resType", C.c_int), ("smoother", C.c_int),
         ("mgTolerance", C.c_double),
+        ("scalar", C.c_int), ("rbcType", C.c_int), ("Ra", C.c_double), ("Pr", C.c_double),
+        ("semantics", C.c_int), ("relTolerance", C.c_double),
     ]
 
 
-VX, VY, VZ, PRES, PCORR, MGRHS, HX, HY, HZ, GX, GY, GZ, DIV = range(13)
+VX, VY, VZ, PRES, PCORR, MGRHS, HX, HY, HZ, GX, GY, GZ, DIV, TEMP, TRHS = range(15)
 
 
 def _ts_bind(L):
@@ -269,13 +271,23 @@ def _ts_bind(L):
     L.or_ts_jacobi_iters.argtypes = [vp, C.c_int]
     L.or_ts_mg_cycles.restype = C.c_int
     L.or_ts_mg_cycles.argtypes = [vp]
+    L.or_ts_init_rbc.argtypes = [vp, C.c_ulonglong, C.c_double]
+    L.or_ts_impose_tbc.argtypes = [vp]
+    L.or_ts_build_trhs.argtypes = [vp]
+    L.or_ts_solve_t.restype = C.c_int
+    L.or_ts_solve_t.argtypes = [vp]
+    L.or_ts_nusselt.restype = C.c_double
+    L.or_ts_nusselt.argtypes = [vp]
     L._ts_bound = True
 
 
 def ts_params(ind=(5, 5, 5), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1.0, 1.0, 1.0), Re=1600.0,
               dt=1e-3, cn_tol=1e-6, depth=None, vc_count=5, pre=4, post=4, res_type=1, smoother=SMOOTH_RB,
-              mg_tol=1e-6, prob_type=2):
+              mg_tol=1e-6, prob_type=2, scalar=False, rbc_type=2, Ra=1e7, Pr=1.0, semantics=SEM_REFERENCE,
+              rel_tol=1e-10):
     p = TsParams()
+    p.scalar, p.rbcType, p.Ra, p.Pr = int(scalar), rbc_type, Ra, Pr
+    p.semantics, p.relTolerance = semantics, rel_tol
     p.xInd, p.yInd, p.zInd = ind
     p.Lx, p.Ly, p.Lz = L
     p.xGrid, p.yGrid, p.zGrid = (1 if c == "U" else 2 for c in mesh)
@@ -367,3 +379,18 @@ class OracleTS:
 
     def mg_cycles(self):
         return self.L.or_ts_mg_cycles(self.h)
+
+    def init_rbc(self, seed=0, amp=1e-3):
+        self.L.or_ts_init_rbc(self.h, seed, amp)
+
+    def impose_tbc(self):
+        self.L.or_ts_impose_tbc(self.h)
+
+    def build_trhs(self):
+        self.L.or_ts_build_trhs(self.h)
+
+    def solve_t(self):
+        return self.L.or_ts_solve_t(self.h)
+
+    def nusselt(self):
+        return self.L.or_ts_nusselt(self.h)
