@@ -76,6 +76,9 @@ int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const double *p2, 
 int sr_guess_train(sr_guess *g, const double *p_star, double *rel_loss);
 /* p_out = 2 p1 - p2 (p1 when p2 is NULL): the extrapolated guess alone, for comparisons */
 int sr_guess_extrapolate(sr_guess *g, const double *p1, const double *p2, double *p_out);
+/* policy 0 (default): sr_guess_predict returns the best of the three guesses by relative
+ * residual; 1: always the learned one (ablation and diagnostics)                               */
+int sr_guess_set_policy(sr_guess *g, int policy);
 /* the correction scale kappa and the number of training steps taken */
 int sr_guess_kappa(const sr_guess *g, double *kappa, int *trained);
 

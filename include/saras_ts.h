@@ -29,6 +29,8 @@ extern "C" {
 #define SR_TS_HY 7
 #define SR_TS_HZ 8
 #define SR_TS_DIV 12
+#define SR_TS_T 13          /* temperature (scalar runs) */
+#define SR_TS_TRHS 14       /* its right-hand side tmpRHS */
 
 typedef struct sr_ts sr_ts;
 
@@ -43,10 +45,15 @@ typedef struct sr_ts_desc {
     double Re, dt, cnTolerance;
     int vcDepth, vcCount, preSmooth, postSmooth, resType, gsSmooth;
     double mgTolerance;
+    int scalar;                     /* 1: scalar_d3 (temperature, buoyancy forcing, forceType 3)  */
+    int rbcType;                    /* scalar.cc:69-86 non-dimensionalisation                      */
+    double Ra, Pr;
+    int semantics;                  /* pressure solve stop rule: SR_SEM_REFERENCE / _PROJECTED     */
+    double relTolerance;
 } sr_ts_desc;
 
 typedef struct sr_ts_stats {
-    int jacobi_iters[3];            /* sweeps of solveVx / solveVy / solveVz                       */
+    int jacobi_iters[4];            /* sweeps of solveVx / solveVy / solveVz / solveT              */
     int mg_cycles;                  /* V-cycles of the pressure solve                              */
     double step_ms;                 /* wall time of the call                                       */
 } sr_ts_stats;
@@ -61,7 +68,8 @@ int sr_ts_upload(sr_ts *ts, int which, const double *host);
 int sr_ts_download(sr_ts *ts, int which, double *host);
 int sr_ts_field_dims(const sr_ts *ts, int which, int *nx, int *ny, int *nz);   /* core sizes */
 
-/* hydro_d3::timeAdvance (hydro_d3.cc:254-368), zero forcing */
+/* hydro_d3::timeAdvance (hydro_d3.cc:254-368), zero forcing; with desc.scalar the step of
+ * scalar_d3::timeAdvance (scalar_d3.cc:243-370) with buoyancy                                   */
 int sr_ts_time_advance(sr_ts *ts, sr_ts_stats *stats);
 
 /* ---- pieces of the step, for parity tests ------------------------------------------------- */
@@ -80,6 +88,20 @@ int sr_ts_divergence(sr_ts *ts);
 /* tseries energy and maximum divergence (lib/io/tseries.cc:137-178) */
 int sr_ts_energy(sr_ts *ts, double *energy);
 int sr_ts_max_div(sr_ts *ts, double *maxdiv);
+
+/* ---- scalar_d3 (src/solvers/scalar/scalar_d3.cc) ------------------------------------------- */
+/* the constructor's state (scalar_d3.cc:85-107: P = 1, T = 0, V = 0) with the temperature
+ * perturbed on the core by amp (2u - 1), u uniform from SplitMix64(seed) in C order; then the
+ * velocity and temperature BCs                                                                   */
+int sr_ts_init_rbc(sr_ts *ts, unsigned long long seed, double amp);
+/* scalar::imposeTBCs (scalar.cc:240-255) with the walls of initTBC for probType 5/6              */
+int sr_ts_impose_tbc(sr_ts *ts);
+/* tmpRHS = T + dt (kappa computeDiff(T) - computeNLin(V, T)), synced (scalar_d3.cc:349-360)     */
+int sr_ts_build_trhs(sr_ts *ts);
+/* scalar_d3::solveT (scalar_d3.cc:533-594)                                                       */
+int sr_ts_solve_t(sr_ts *ts, int *iters);
+/* tseries Nusselt number 1 + <w T>/kappa (tseries.cc:195-246)                                    */
+int sr_ts_nusselt(sr_ts *ts, double *nu);
 
 #ifdef __cplusplus
 }

@@ -576,8 +576,10 @@ void or_ts_impose_tbc(or_ts *t) {
         wall_bc(f, 1, F, 5, 0, 0.0);
         wall_bc(f, 1, F, 4, 0, 0.0);
     } else {
-        wall_bc(f, 0, F, 5, 2, 0.0);
-        wall_bc(f, 0, F, 4, 2, 1.0);
+        /* probType 5: hot bottom, cold top; probType 6 the reverse (scalar.cc:206-227) */
+        double top = t->p.probType == 6 ? 1.0 : 0.0, bottom = t->p.probType == 6 ? 0.0 : 1.0;
+        wall_bc(f, 0, F, 5, 2, top);
+        wall_bc(f, 0, F, 4, 2, bottom);
     }
 }
 

@@ -48,7 +48,9 @@ def lib(fast=False):
     if name in _libs:
         return _libs[name]
     path = os.path.join(_BUILD, name)
-    if not os.path.exists(path):
+    srcs = [os.path.join(os.path.dirname(os.path.abspath(__file__)), f)
+            for f in ("mg_oracle.c", "field_oracle.c", "oracle.h")]
+    if not os.path.exists(path) or any(os.path.getmtime(f) > os.path.getmtime(path) for f in srcs if os.path.exists(f)):
         build()
     L = C.CDLL(path)
     dp = C.POINTER(C.c_double)

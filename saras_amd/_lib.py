@@ -38,7 +38,7 @@ class SrMgStats(C.Structure):
 
 
 SR_TS_VX, SR_TS_VY, SR_TS_VZ, SR_TS_P, SR_TS_PP, SR_TS_MGRHS = 0, 1, 2, 3, 4, 5
-SR_TS_HX, SR_TS_HY, SR_TS_HZ, SR_TS_DIV = 6, 7, 8, 12
+SR_TS_HX, SR_TS_HY, SR_TS_HZ, SR_TS_DIV, SR_TS_T, SR_TS_TRHS = 6, 7, 8, 12, 13, 14
 
 
 class SrTsDesc(C.Structure):
@@ -53,11 +53,13 @@ class SrTsDesc(C.Structure):
         ("vcDepth", C.c_int), ("vcCount", C.c_int), ("preSmooth", C.c_int), ("postSmooth", C.c_int),
         ("resType", C.c_int), ("gsSmooth", C.c_int),
         ("mgTolerance", C.c_double),
+        ("scalar", C.c_int), ("rbcType", C.c_int), ("Ra", C.c_double), ("Pr", C.c_double),
+        ("semantics", C.c_int), ("relTolerance", C.c_double),
     ]
 
 
 class SrTsStats(C.Structure):
-    _fields_ = [("jacobi_iters", C.c_int * 3), ("mg_cycles", C.c_int), ("step_ms", C.c_double)]
+    _fields_ = [("jacobi_iters", C.c_int * 4), ("mg_cycles", C.c_int), ("step_ms", C.c_double)]
 
 
 SR_GUESS_LAYERS, SR_GUESS_IN_CH = 12, 8
@@ -120,6 +122,11 @@ SIGNATURES = [
     ("sr_ts_divergence", C.c_int, [_vp]),
     ("sr_ts_energy", C.c_int, [_vp, _dp]),
     ("sr_ts_max_div", C.c_int, [_vp, _dp]),
+    ("sr_ts_init_rbc", C.c_int, [_vp, C.c_ulonglong, C.c_double]),
+    ("sr_ts_impose_tbc", C.c_int, [_vp]),
+    ("sr_ts_build_trhs", C.c_int, [_vp]),
+    ("sr_ts_solve_t", C.c_int, [_vp, _ip]),
+    ("sr_ts_nusselt", C.c_int, [_vp, _dp]),
     ("sr_guess_create", C.c_int, [C.POINTER(SrGuessDesc), C.POINTER(C.c_void_p)]),
     ("sr_guess_destroy", C.c_int, [_vp]),
     ("sr_guess_layer_info", C.c_int, [C.c_int, _ip, _ip, _ip]),
@@ -135,6 +142,7 @@ SIGNATURES = [
     ("sr_guess_train", C.c_int, [_vp, C.c_void_p, _dp]),
     ("sr_guess_kappa", C.c_int, [_vp, _dp, _ip]),
     ("sr_guess_extrapolate", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("sr_guess_set_policy", C.c_int, [_vp, C.c_int]),
 ]
 
 _lib = None

@@ -304,8 +304,13 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
         rel_learned = rel_ex;   // untrained: the correction is zero, the guess is the extrapolation
         choice = 1;
     }
-    if (choice == 2 && !(rel_learned < rel_ex && rel_learned < rel_warm)) choice = rel_ex <= rel_warm ? 1 : 0;
-    if (choice == 1 && rel_warm < rel_ex) choice = 0;
+    if (g->policy == 1) {
+        if (g->kappa <= 0.0) RETCHK(sr_mg_residual(m, p_out, rhs, nullptr, &rel_learned));
+        choice = 2;
+    } else {
+        if (choice == 2 && !(rel_learned < rel_ex && rel_learned < rel_warm)) choice = rel_ex <= rel_warm ? 1 : 0;
+        if (choice == 1 && rel_warm < rel_ex) choice = 0;
+    }
     if (choice != 2) {
         const double *src = choice == 1 ? g->extrap : p1;
         HIPCHK(hipMemcpyAsync(p_out, src, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
@@ -362,6 +367,12 @@ extern "C" int sr_guess_extrapolate(sr_guess *g, const double *p1, const double 
     hipLaunchKernelGGL(k_extrap, dim3(nblk(g->fine_pad)), dim3(256), 0, g->s, g->fine_pad, p1, p2, p_out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(g->s));
+    return SR_OK;
+}
+
+extern "C" int sr_guess_set_policy(sr_guess *g, int policy) {
+    if (!g || policy < 0 || policy > 1) return set_err(SR_ERR_ARG, "policy must be 0 or 1");
+    g->policy = policy;
     return SR_OK;
 }
 

@@ -11,7 +11,7 @@ namespace sr {
 template <typename T> void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s);
 template <typename T>
 void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
-                     const Bc &bc, bool zero_start, hipStream_t s, int one_per_thread = 0);
+                     const Bc &bc, bool zero_start, hipStream_t s);
 template <typename T, typename TO>
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
                      hipStream_t s);

@@ -199,73 +199,6 @@ __global__ void __launch_bounds__(256) k_rb_sweep(Lay L, Geo G, T *__restrict__ 
     write_images(L, u, ip, jp, kp, v, bc);
 }
 
-// Two adjacent entries h0, h0 + 1 of one half-row per thread (same row parity kb): the x/y
-// neighbours and f arrive as 16-byte pairs, the z neighbours as three consecutive entries shared by
-// the pair.  Requires an even half-row stride (16-byte aligned pairs).
-template <typename T> struct Pair;
-template <> struct Pair<double> { typedef double2 type; };
-template <> struct Pair<float> { typedef float2 type; };
-
-template <typename T, bool ZERO_START>
-__global__ void __launch_bounds__(256) k_rb_sweep2(Lay L, Geo G, T *__restrict__ u, const T *__restrict__ f,
-                                                   const T *__restrict__ cx, const T *__restrict__ cy,
-                                                   const T *__restrict__ cz, int color, Bc bc) {
-    typedef typename Pair<T>::type T2;
-    unsigned ipu, r;
-    if (!geo_slot(G, xcd_block(), ipu, r)) return;
-    int ip = (int)ipu;
-    unsigned jr = fdivu(r, G.d1, G.inv1);
-    int jp = 1 + (int)jr;
-    int h0 = 2 * (int)(r - jr * G.d1);
-    int kb = (color + 1 + ip + jp) & 1;
-    int kpa = 2 * h0 + kb, kpb = kpa + 2;
-    bool va = kpa >= 1 && kpa <= L.nz, vb = kpb >= 1 && kpb <= L.nz;
-    if (!va && !vb) return;
-    long rc = lay_row(L, ip, jp, color);
-    const T *X = cx + 4 * ip, *Y = cy + 4 * jp;
-    const T *Za = cz + 4 * (va ? kpa : kpb), *Zb = cz + 4 * (vb ? kpb : kpa);
-    T2 fv = *(const T2 *)(f + rc + h0);
-    T na, nb;
-    if (ZERO_START) {
-        na = -fv.x;
-        nb = -fv.y;
-    } else {
-        long ro = lay_row(L, ip, jp, color ^ 1);
-        long sx = 2L * (L.ny + 2) * L.khs;
-        long sy = 2L * L.khs;
-        T2 xm = *(const T2 *)(u + ro - sx + h0), xp = *(const T2 *)(u + ro + sx + h0);
-        T2 ym = *(const T2 *)(u + ro - sy + h0), yp = *(const T2 *)(u + ro + sy + h0);
-        T z0, z1, z2;
-        if (kb) {
-            T2 a = *(const T2 *)(u + ro + h0);
-            z0 = a.x;
-            z1 = a.y;
-            z2 = u[ro + h0 + 2];
-        } else {
-            z0 = u[ro + h0 - 1];
-            T2 a = *(const T2 *)(u + ro + h0);
-            z1 = a.x;
-            z2 = a.y;
-        }
-        na = X[0] * xp.x + X[1] * xm.x + Y[0] * yp.x + Y[1] * ym.x + Za[0] * z1 + Za[1] * z0 - fv.x;
-        nb = X[0] * xp.y + X[1] * xm.y + Y[0] * yp.y + Y[1] * ym.y + Zb[0] * z2 + Zb[1] * z1 - fv.y;
-    }
-    T a = na / (X[2] + Y[2] + Za[2]);
-    T b = nb / (X[2] + Y[2] + Zb[2]);
-    if (va && vb) {
-        T2 w;
-        w.x = a;
-        w.y = b;
-        *(T2 *)(u + rc + h0) = w;
-    } else if (va) {
-        u[rc + h0] = a;
-    } else {
-        u[rc + h0 + 1] = b;
-    }
-    if (va) write_images(L, u, ip, jp, kpa, a, bc);
-    if (vb) write_images(L, u, ip, jp, kpb, b, bc);
-}
-
 // ------------------------------------------------------------------------------------------------
 // residual t = f - L u on core nodes (both colours)
 // ------------------------------------------------------------------------------------------------
@@ -299,61 +232,6 @@ __global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restr
     long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
     T lu = apply_lap<T, TO>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
     t_out[rc + h] = (TO)(f[rc + h] - lu);
-}
-
-// The residual with two adjacent half-row entries per thread (slots (jp, c, pair)); even stride.
-template <typename T, typename TO>
-__global__ void __launch_bounds__(256) k_residual2(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
-                                                   TO *__restrict__ t_out, const T *__restrict__ cx,
-                                                   const T *__restrict__ cy, const T *__restrict__ cz) {
-    typedef typename Pair<T>::type T2;
-    typedef typename Pair<TO>::type TO2;
-    unsigned ipu, rr;
-    if (!geo_slot(G, xcd_block(), ipu, rr)) return;
-    int ip = (int)ipu;
-    unsigned jr = fdivu(rr, G.d1, G.inv1);
-    unsigned r2 = rr - jr * G.d1;
-    int jp = 1 + (int)jr;
-    const unsigned kp2 = G.d1 >> 1;
-    int c = r2 >= kp2 ? 1 : 0;
-    int h0 = 2 * (int)(r2 - (unsigned)c * kp2);
-    int kb = (c + 1 + ip + jp) & 1;
-    int kpa = 2 * h0 + kb, kpb = kpa + 2;
-    bool va = kpa >= 1 && kpa <= L.nz, vb = kpb >= 1 && kpb <= L.nz;
-    if (!va && !vb) return;
-    long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
-    const T *X = cx + 4 * ip, *Y = cy + 4 * jp;
-    const T *Za = cz + 4 * (va ? kpa : kpb), *Zb = cz + 4 * (vb ? kpb : kpa);
-    long sx = 2L * (L.ny + 2) * L.khs;
-    long sy = 2L * L.khs;
-    T2 uc = *(const T2 *)(u + rc + h0), fv = *(const T2 *)(f + rc + h0);
-    T2 xm = *(const T2 *)(u + ro - sx + h0), xp = *(const T2 *)(u + ro + sx + h0);
-    T2 ym = *(const T2 *)(u + ro - sy + h0), yp = *(const T2 *)(u + ro + sy + h0);
-    T z0, z1, z2;
-    if (kb) {
-        T2 a = *(const T2 *)(u + ro + h0);
-        z0 = a.x;
-        z1 = a.y;
-        z2 = u[ro + h0 + 2];
-    } else {
-        z0 = u[ro + h0 - 1];
-        T2 a = *(const T2 *)(u + ro + h0);
-        z1 = a.x;
-        z2 = a.y;
-    }
-    T la = X[0] * xp.x + X[1] * xm.x + Y[0] * yp.x + Y[1] * ym.x + Za[0] * z1 + Za[1] * z0 - (X[2] + Y[2] + Za[2]) * uc.x;
-    T lb = X[0] * xp.y + X[1] * xm.y + Y[0] * yp.y + Y[1] * ym.y + Zb[0] * z2 + Zb[1] * z1 - (X[2] + Y[2] + Zb[2]) * uc.y;
-    TO ta = (TO)(fv.x - la), tb = (TO)(fv.y - lb);
-    if (va && vb) {
-        TO2 w;
-        w.x = ta;
-        w.y = tb;
-        *(TO2 *)(t_out + rc + h0) = w;
-    } else if (va) {
-        t_out[rc + h0] = ta;
-    } else {
-        t_out[rc + h0 + 1] = tb;
-    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -557,75 +435,6 @@ __global__ void __launch_bounds__(256) k_norm_partial(Lay L, Geo G, const T *__r
             double v = fabs(lu - fv);
             s0 += v * v;
             s1 += fv * fv;
-            m2 = fmax(m2, v);
-        }
-    }
-    block_reduce4(s0, s1, m2, m3, part);
-}
-
-// The same four quantities with two adjacent stored entries per slot (even stride): slots
-// (jp, c, pair) over every padded column.
-template <typename T>
-__global__ void __launch_bounds__(256) k_norm_partial2(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
-                                                       const T *__restrict__ cx, const T *__restrict__ cy,
-                                                       const T *__restrict__ cz, double *__restrict__ part) {
-    typedef typename Pair<T>::type T2;
-    double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
-    unsigned lb0, lbs, lbe;
-    xcd_range(G.total_blocks, lb0, lbs, lbe);
-    const unsigned kp2 = G.d1 >> 1;
-    const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
-    for (unsigned lb = lb0; lb < lbe; lb += lbs) {
-        unsigned ipu, r;
-        if (!geo_slot(G, lb, ipu, r)) continue;
-        unsigned jpu = fdivu(r, G.d1, G.inv1);
-        unsigned r2 = r - jpu * G.d1;
-        int c = r2 >= kp2 ? 1 : 0;
-        int h0 = 2 * (int)(r2 - (unsigned)c * kp2);
-        int ip = (int)ipu, jp = (int)jpu;
-        int kb = (c + 1 + ip + jp) & 1;
-        int kpa = 2 * h0 + kb, kpb = kpa + 2;
-        bool sa = h0 < L.kh && kpa <= L.nz + 1, sb = h0 + 1 < L.kh && kpb <= L.nz + 1;
-        if (!sa && !sb) continue;
-        long rc = lay_row(L, ip, jp, c);
-        T2 fv = *(const T2 *)(f + rc + h0);
-        if (sa) m3 = fmax(m3, fabs((double)fv.x));
-        if (sb) m3 = fmax(m3, fabs((double)fv.y));
-        if (ip < 1 || ip > L.nx || jp < 1 || jp > L.ny) continue;
-        bool va = kpa >= 1 && kpa <= L.nz, vb = kpb >= 1 && kpb <= L.nz;
-        if (!va && !vb) continue;
-        long ro = lay_row(L, ip, jp, c ^ 1);
-        const T *X = cx + 4 * ip, *Y = cy + 4 * jp;
-        const T *Za = cz + 4 * (va ? kpa : kpb), *Zb = cz + 4 * (vb ? kpb : kpa);
-        T2 uc = *(const T2 *)(u + rc + h0);
-        T2 xm = *(const T2 *)(u + ro - sx + h0), xp = *(const T2 *)(u + ro + sx + h0);
-        T2 ym = *(const T2 *)(u + ro - sy + h0), yp = *(const T2 *)(u + ro + sy + h0);
-        T z0, z1, z2;
-        if (kb) {
-            T2 a = *(const T2 *)(u + ro + h0);
-            z0 = a.x;
-            z1 = a.y;
-            z2 = u[ro + h0 + 2];
-        } else {
-            z0 = u[ro + h0 - 1];
-            T2 a = *(const T2 *)(u + ro + h0);
-            z1 = a.x;
-            z2 = a.y;
-        }
-        if (va) {
-            double lu = (double)(X[0] * xp.x + X[1] * xm.x + Y[0] * yp.x + Y[1] * ym.x + Za[0] * z1 + Za[1] * z0 -
-                                 (X[2] + Y[2] + Za[2]) * uc.x);
-            double v = fabs(lu - (double)fv.x);
-            s0 += v * v;
-            s1 += (double)fv.x * (double)fv.x;
-            m2 = fmax(m2, v);
-        }
-        if (vb) {
-            double lu = (double)(X[0] * xp.y + X[1] * xm.y + Y[0] * yp.y + Y[1] * ym.y + Zb[0] * z2 + Zb[1] * z1 -
-                                 (X[2] + Y[2] + Zb[2]) * uc.y);
-            double v = fabs(lu - (double)fv.y);
-            s0 += v * v;
-            s1 += (double)fv.y * (double)fv.y;
             m2 = fmax(m2, v);
         }
     }
@@ -1147,18 +956,7 @@ void launch_fill_ghosts(const Lay &L, T *a, const Bc &bc, hipStream_t s) {
 
 template <typename T>
 void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
-                     const Bc &bc, bool zero_start, hipStream_t s, int one_per_thread) {
-    if (!one_per_thread && (L.khs % 2) == 0) {
-        unsigned kp2 = (unsigned)((L.kh + 1) / 2);
-        Geo G = geo_planes((unsigned)L.ny * kp2, 1, L.nx, kp2);
-        if (zero_start)
-            hipLaunchKernelGGL((k_rb_sweep2<T, true>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, cx, cy, cz,
-                               color, bc);
-        else
-            hipLaunchKernelGGL((k_rb_sweep2<T, false>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, cx, cy, cz,
-                               color, bc);
-        return;
-    }
+                     const Bc &bc, bool zero_start, hipStream_t s) {
     Geo G = geo_sweep(L);
     if (zero_start)
         hipLaunchKernelGGL((k_rb_sweep<T, true>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, cx, cy, cz, color, bc);
@@ -1169,12 +967,6 @@ void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, c
 template <typename T, typename TO>
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
                      hipStream_t s) {
-    if ((L.khs % 2) == 0) {
-        unsigned kp2 = (unsigned)((L.kh + 1) / 2);
-        Geo G = geo_planes((unsigned)L.ny * 2u * kp2, 1, L.nx, 2u * kp2);
-        hipLaunchKernelGGL((k_residual2<T, TO>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz);
-        return;
-    }
     Geo G = geo_core(L);
     hipLaunchKernelGGL((k_residual<T, TO>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz);
 }
@@ -1222,14 +1014,6 @@ int norm_parts(const Lay &L) { return (int)reduce_grid(geo_all(L)); }
 template <typename T>
 void launch_norm(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part,
                  double *out, hipStream_t s) {
-    if ((L.khs % 2) == 0) {
-        unsigned kp2 = (unsigned)(L.khs / 2);
-        Geo G = geo_planes((unsigned)(L.ny + 2) * 2u * kp2, 0, L.nx + 2, 2u * kp2);
-        int nb = (int)reduce_grid(G);
-        hipLaunchKernelGGL(k_norm_partial2<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part);
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
-        return;
-    }
     Geo G = geo_all(L);
     int nb = (int)reduce_grid(G);
     hipLaunchKernelGGL(k_norm_partial<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part);
@@ -1314,7 +1098,7 @@ int coarse_cycle_attr(size_t lds) {
 #define SR_INST(T)                                                                                          \
     template void launch_fill_ghosts<T>(const Lay &, T *, const Bc &, hipStream_t);                        \
     template void launch_rb_sweep<T>(const Lay &, T *, const T *, const T *, const T *, const T *, int,     \
-                                     const Bc &, bool, hipStream_t, int);                                        \
+                                     const Bc &, bool, hipStream_t);                                        \
     template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t);               \
     template void launch_fill<T>(T *, long, T, hipStream_t);                                               \
     template void launch_add_scalar<T>(const Lay &, T *, const double *, double, int, hipStream_t);        \

@@ -347,8 +347,7 @@ extern "C" int sr_mg_kernel_bytes(const sr_mg *m, int kernel, int level, double 
             break;
         }
         case 5: *bytes = 2.0 * nn * w; break;                     // norm: read u, f
-        case 6: *bytes = 12.0 * nn / 8.0 * w; break;              // half sweep, one node per thread
-        case 7: *bytes = 12.0 * nn / 8.0 * 4.0; break;            // FP32 half sweep
+        case 6: *bytes = 12.0 * nn / 8.0 * 4.0; break;            // FP32 half sweep
         default: return set_err(SR_ERR_ARG, "unknown kernel id");
     }
     return SR_OK;
@@ -369,15 +368,12 @@ extern "C" int sr_mg_time_kernel(sr_mg *m, int kernel, int level, int reps, doub
             case 4: launch_fill_ghosts<double>(V.L, V.u, bc, s); break;
             case 5: launch_norm<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part, m->dres, s); break;
             case 6:
-                launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], r & 1, bc, false, s, 1);
-                break;
-            case 7:
                 launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], r & 1, bc, false, s);
                 break;
         }
     };
-    if (kernel < 0 || kernel > 7) return set_err(SR_ERR_ARG, "unknown kernel id");
-    if (kernel == 7 && !V.u32) return set_err(SR_ERR_STATE, "no FP32 hierarchy (mixed precision not enabled)");
+    if (kernel < 0 || kernel > 6) return set_err(SR_ERR_ARG, "unknown kernel id");
+    if (kernel == 6 && !V.u32) return set_err(SR_ERR_STATE, "no FP32 hierarchy (mixed precision not enabled)");
     launch(0);
     launch(1);
     HIPCHK(hipEventRecord(m->ev0, s));

@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -48,8 +49,13 @@ struct sr_ts {
     GhostWave sync_v[3]{}, sync_p{};             // syncData of the velocity components and of P
     sr_mg *mg = nullptr;
     hipStream_t s = nullptr;
-    int jac[3] = {0, 0, 0};
+    int jac[4] = {0, 0, 0, 0};
     int mg_cycles = 0;
+    // scalar_d3
+    double *T = nullptr, *Trhs = nullptr, *Tg = nullptr;
+    std::vector<GhostWave> tbc;
+    GhostWave sync_t{};
+    double nu = 0.0, kappa = 0.0, Fb = 0.0;
 };
 
 static long fsize(const FShape &f) { return (long)(f.n[0] + 2) * (f.n[1] + 2) * (f.n[2] + 2); }
@@ -177,6 +183,37 @@ static void impose_vbc(sr_ts *t, int c) {
     for (const GhostWave &w : t->vbc[c]) ts_ghost_wave(t->fv[c], t->V[c], w, t->s);
 }
 
+// scalar::imposeTBCs (scalar.cc:240-255) with the walls of initTBC (scalar.cc:196-228): syncData,
+// adiabatic (neumannCC) x and y walls, then top and bottom plates (probType 5: cold top, hot
+// bottom; 6: the reverse), or periodicCC in z
+static std::vector<GhostWave> tbc_waves(const sr_ts_desc &d, const FShape &f) {
+    std::vector<GhostWave> ws;
+    ws.push_back(sync_wave(f, d.xPer, d.yPer));
+    int per[3] = {d.xPer, d.yPer, d.zPer};
+    for (int dim = 0; dim < 2; dim++) {
+        if (per[dim]) continue;
+        GhostWave w{};
+        add_op(w, wall_op(f, 0, 2 * dim, 4, 0.0));
+        add_op(w, wall_op(f, 0, 2 * dim + 1, 4, 0.0));
+        ws.push_back(w);
+    }
+    GhostWave w{};
+    if (per[2]) {
+        add_op(w, wall_op(f, 1, 5, 0, 0.0));
+        add_op(w, wall_op(f, 1, 4, 0, 0.0));
+    } else {
+        double top = d.probType == 6 ? 1.0 : 0.0, bottom = d.probType == 6 ? 0.0 : 1.0;
+        add_op(w, wall_op(f, 0, 5, 2, top));
+        add_op(w, wall_op(f, 0, 4, 2, bottom));
+    }
+    ws.push_back(w);
+    return ws;
+}
+
+static void impose_tbc(sr_ts *t) {
+    for (const GhostWave &w : t->tbc) ts_ghost_wave(t->fp, t->T, w, t->s);
+}
+
 // ------------------------------------------------------------------------------------------------
 // construction
 // ------------------------------------------------------------------------------------------------
@@ -191,8 +228,21 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
         if (ind[k] < 2 || ind[k] > 12) return set_err(SR_ERR_ARG, "grid size index out of range 2..12");
         if (L[k] <= 0.0) return set_err(SR_ERR_ARG, "domain length must be positive");
     }
-    if (d.Re <= 0.0 || d.dt <= 0.0 || d.cnTolerance <= 0.0)
-        return set_err(SR_ERR_ARG, "Re, dt and cnTolerance must be positive");
+    if (d.dt <= 0.0 || d.cnTolerance <= 0.0) return set_err(SR_ERR_ARG, "dt and cnTolerance must be positive");
+    if (!d.scalar && d.Re <= 0.0) return set_err(SR_ERR_ARG, "Re must be positive");
+    if (d.scalar) {
+        if (d.Ra <= 0.0 || d.Pr <= 0.0) return set_err(SR_ERR_ARG, "Ra and Pr must be positive");
+        // scalar.cc:69-86, buoyantForce.cc:45-55
+        switch (d.rbcType) {
+            case 1: t->nu = d.Pr; t->kappa = 1.0; t->Fb = d.Ra * d.Pr; break;
+            case 2: t->nu = std::sqrt(d.Pr / d.Ra); t->kappa = 1.0 / std::sqrt(d.Pr * d.Ra); t->Fb = 1.0; break;
+            case 3: t->nu = 1.0; t->kappa = 1.0 / d.Pr; t->Fb = d.Ra; break;
+            case 4: t->nu = std::sqrt(d.Pr / d.Ra); t->kappa = 1.0 / std::sqrt(d.Pr * d.Ra); t->Fb = d.Pr; break;
+            default: return set_err(SR_ERR_ARG, "rbcType must be 1..4");
+        }
+    } else {
+        t->nu = 1.0 / d.Re;
+    }
     for (int k = 0; k < 3; k++) {
         int n = (1 << ind[k]) + 1;
         t->N[k] = n;
@@ -237,10 +287,10 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
     md.preSmooth = d.preSmooth; md.postSmooth = d.postSmooth;
     md.gsSmooth = d.gsSmooth; md.resType = d.resType;
     md.xPer = d.xPer; md.yPer = d.yPer; md.zPer = d.zPer;
-    md.semantics = SR_SEM_REFERENCE;
+    md.semantics = d.semantics;
     md.precision = SR_PREC_FP64;
     md.mgTolerance = d.mgTolerance;
-    md.relTolerance = 1e-10;
+    md.relTolerance = d.relTolerance > 0.0 ? d.relTolerance : 1e-10;
     md.dXi = t->h[0]; md.dEt = t->h[1]; md.dZt = t->h[2];
     md.xix2 = t->hmet[0][0][3].data(); md.xixx = t->hmet[0][0][2].data();
     md.ety2 = t->hmet[1][0][3].data(); md.etyy = t->hmet[1][0][2].data();
@@ -269,6 +319,15 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
     ts_fill(t->Pp, np, 0.0, t->s);
     ts_fill(t->mgRHS, np, 0.0, t->s);
     ts_fill(t->divV, np, 0.0, t->s);
+    if (d.scalar) {
+        HIPCHK(hipMalloc((void **)&t->T, sizeof(double) * np));
+        HIPCHK(hipMalloc((void **)&t->Trhs, sizeof(double) * np));
+        HIPCHK(hipMalloc((void **)&t->Tg, sizeof(double) * np));
+        ts_fill(t->T, np, 0.0, t->s);
+        ts_fill(t->Trhs, np, 0.0, t->s);
+        ts_fill(t->Tg, np, 0.0, t->s);
+        t->tbc = tbc_waves(d, t->fp);
+    }
     HIPCHK(hipMalloc((void **)&t->part, sizeof(double) * 2 * (size_t)ts_energy_parts(t->fp)));
     HIPCHK(hipMalloc((void **)&t->dred, sizeof(double) * 4));
     HIPCHK(hipHostMalloc((void **)&t->hred, sizeof(double) * 4, hipHostMallocDefault));
@@ -287,7 +346,7 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
 extern "C" int sr_ts_destroy(sr_ts *t) {
     if (!t) return SR_OK;
     if (t->s) (void)hipStreamSynchronize(t->s);
-    double *bufs[] = {t->dmet, t->P, t->Pp, t->mgRHS, t->divV, t->scratch, t->part, t->dred};
+    double *bufs[] = {t->dmet, t->P, t->Pp, t->mgRHS, t->divV, t->scratch, t->part, t->dred, t->T, t->Trhs, t->Tg};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     for (int c = 0; c < 3; c++) {
@@ -330,6 +389,12 @@ static int field(sr_ts *t, int which, double **ptr, const FShape **shape) {
         case SR_TS_HX: case SR_TS_HY: case SR_TS_HZ:
             *ptr = t->H[which - SR_TS_HX]; *shape = &t->fv[which - SR_TS_HX]; return SR_OK;
         case SR_TS_DIV: *ptr = t->divV; *shape = &t->fp; return SR_OK;
+        case SR_TS_T:
+        case SR_TS_TRHS:
+            if (!t->T) return set_err(SR_ERR_STATE, "no temperature field (desc.scalar = 0)");
+            *ptr = which == SR_TS_T ? t->T : t->Trhs;
+            *shape = &t->fp;
+            return SR_OK;
         default: return set_err(SR_ERR_ARG, "unknown field");
     }
 }
@@ -402,8 +467,11 @@ extern "C" int sr_ts_deriv(sr_ts *t, int which, int order, int dim, double *out)
 
 static void build_rhs(sr_ts *t) {
     const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    // hydro_d3 multiplies the diffusion by 1/Re (hydro_d3.cc:259-262), scalar_d3 by nu
+    const double visc = t->d.scalar ? t->nu : 1.0 / t->d.Re;
     for (int c = 0; c < 3; c++) {
-        ts_build_rhs(c, t->fv, t->fp, t->M, Vc, t->P, t->H[c], t->h, 1.0 / t->d.Re, t->d.dt, t->s);
+        ts_build_rhs(c, t->fv, t->fp, t->M, Vc, t->P, t->H[c], t->h, visc, t->d.dt, t->d.scalar ? t->T : nullptr,
+                     t->Fb, t->s);
         ts_ghost_wave(t->fv[c], t->H[c], t->sync_v[c], t->s);
     }
 }
@@ -416,10 +484,25 @@ extern "C" int sr_ts_build_rhs(sr_ts *t) {
     return SR_OK;
 }
 
-// hydro_d3::solveVx/Vy/Vz (hydro_d3.cc:370-533): Jacobi sweeps until the maximum residual is below
-// cnTolerance; the reference aborts after (Nx-1)(Ny-1)(Nz-1) sweeps without convergence.
-static int solve_v(sr_ts *t, int c, int *iters) {
-    const FShape &f = t->fv[c];
+static JacobiCoef jacobi_coef(const sr_ts *t) {
+    JacobiCoef J{};
+    J.hx = t->h[0]; J.hy = t->h[1]; J.hz = t->h[2];
+    J.hx2hy2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0);
+    J.hy2hz2 = std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
+    J.hz2hx2 = std::pow(J.hz, 2.0) * std::pow(J.hx, 2.0);
+    J.hx2hy2hz2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
+    J.dt = t->d.dt;
+    J.Re = t->d.Re;
+    J.inverseRe = t->d.Re > 0.0 ? 1.0 / t->d.Re : 0.0;
+    return J;
+}
+
+// Jacobi sweeps of (1 - dt vis lap / 2) F = R until the maximum residual over the bulk is below
+// cnTolerance; the reference aborts after (Nx-1)(Ny-1)(Nz-1) sweeps without convergence.  F and G
+// swap after every sweep (G is written everywhere: 0 outside the bulk, as the reference's guess
+// field), then the field's BCs are imposed.
+static int jacobi(sr_ts *t, const FShape &f, double *&F, double *&G, const double *R, const JacobiCoef &J,
+                  const std::vector<GhostWave> &bc, const char *what, int *iters) {
     int per[3] = {t->d.xPer, t->d.yPer, t->d.zPer};
     Bulk B{};
     for (int k = 0; k < 3; k++) {
@@ -430,24 +513,15 @@ static int solve_v(sr_ts *t, int c, int *iters) {
             B.hi[k] = f.n[k] - 2;
         }
     }
-    JacobiCoef J{};
-    J.hx = t->h[0]; J.hy = t->h[1]; J.hz = t->h[2];
-    J.hx2hy2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0);
-    J.hy2hz2 = std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
-    J.hz2hx2 = std::pow(J.hz, 2.0) * std::pow(J.hx, 2.0);
-    J.hx2hy2hz2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
-    J.dt = t->d.dt;
-    J.Re = t->d.Re;
-    J.inverseRe = 1.0 / t->d.Re;
     long maxIterations = (long)(t->N[0] - 1) * (t->N[1] - 1) * (t->N[2] - 1);
     long iterCount = 0;
     int sweeps = 0;
     while (true) {
-        ts_jacobi(f, t->M, B, t->V[c], t->H[c], t->G[c], J, t->s);
-        std::swap(t->V[c], t->G[c]);
-        impose_vbc(t, c);
+        ts_jacobi(f, t->M, B, F, R, G, J, t->s);
+        std::swap(F, G);
+        for (const GhostWave &w : bc) ts_ghost_wave(f, F, w, t->s);
         ts_clear_u64(t->dmax, t->s);
-        ts_jacobi_res(f, t->M, B, t->V[c], t->H[c], J, t->dmax, t->s);
+        ts_jacobi_res(f, t->M, B, F, R, J, t->dmax, t->s);
         HIPCHK(hipMemcpyAsync(t->hmax, t->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, t->s));
         HIPCHK(hipStreamSynchronize(t->s));
         HIPCHK(hipGetLastError());
@@ -458,10 +532,39 @@ static int solve_v(sr_ts *t, int c, int *iters) {
         if (maxError < t->d.cnTolerance) break;
         iterCount += 1;
         if (iterCount > maxIterations)
-            return set_err(SR_ERR_NOT_CONVERGED, "Jacobi viscous solve did not converge within the iteration limit");
+            return set_err(SR_ERR_NOT_CONVERGED, std::string("Jacobi iterations for ") + what +
+                                                     " did not converge within the iteration limit");
     }
-    t->jac[c] = sweeps;
     if (iters) *iters = sweeps;
+    return SR_OK;
+}
+
+// hydro_d3::solveVx/Vy/Vz (hydro_d3.cc:370-533), scalar_d3::solveVx/Vy/Vz (scalar_d3.cc:373-531)
+static int solve_v(sr_ts *t, int c, int *iters) {
+    JacobiCoef J = jacobi_coef(t);
+    J.form = t->d.scalar ? 1 : 0;
+    J.vis = t->nu;
+    for (int k = 0; k < 3; k++) J.ms[k] = t->fv[c].s[k];
+    int n = 0;
+    RETCHK(jacobi(t, t->fv[c], t->V[c], t->G[c], t->H[c], J, t->vbc[c], c == 0 ? "Vx" : c == 1 ? "Vy" : "Vz", &n));
+    t->jac[c] = n;
+    if (iters) *iters = n;
+    return SR_OK;
+}
+
+// scalar_d3::solveT (scalar_d3.cc:533-594): node field; the z metric is the collocated xi_z^2
+// array read at node indices, as the reference writes it (scalar_d3.cc:541, 561)
+static int solve_t(sr_ts *t, int *iters) {
+    JacobiCoef J = jacobi_coef(t);
+    J.form = 1;
+    J.vis = t->kappa;
+    J.ms[0] = 1;
+    J.ms[1] = 1;
+    J.ms[2] = 0;
+    int n = 0;
+    RETCHK(jacobi(t, t->fp, t->T, t->Tg, t->Trhs, J, t->tbc, "T", &n));
+    t->jac[3] = n;
+    if (iters) *iters = n;
     return SR_OK;
 }
 
@@ -521,14 +624,93 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     ts_ghost_wave(t->fp, t->Pp, t->sync_p, t->s);
     ts_axpy_full(t->P, t->Pp, fsize(t->fp), t->s);
     for (int c = 0; c < 3; c++) ts_project(c, t->fv[c], t->fp, t->M, t->Pp, t->V[c], t->h[c], dt, t->s);
+    if (t->d.scalar) {
+        // the temperature equation with the projected velocity (scalar_d3.cc:349-363)
+        const double *Vp[3] = {t->V[0], t->V[1], t->V[2]};
+        ts_build_trhs(t->fp, t->fv, t->M, Vp, t->T, t->Trhs, t->h, t->kappa, dt, t->s);
+        ts_ghost_wave(t->fp, t->Trhs, t->sync_p, t->s);
+        RETCHK(solve_t(t, nullptr));
+    }
     for (int c = 0; c < 3; c++) impose_vbc(t, c);
+    if (t->d.scalar) impose_tbc(t);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(t->s));
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
-        for (int c = 0; c < 3; c++) st->jacobi_iters[c] = t->jac[c];
+        for (int c = 0; c < 4; c++) st->jacobi_iters[c] = t->jac[c];
         st->mg_cycles = t->mg_cycles;
         st->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// scalar_d3 entry points
+// ------------------------------------------------------------------------------------------------
+static int need_scalar(const sr_ts *t) {
+    if (!t) return set_err(SR_ERR_ARG, "null argument");
+    if (!t->T) return set_err(SR_ERR_STATE, "no temperature field (desc.scalar = 0)");
+    return SR_OK;
+}
+
+extern "C" int sr_ts_init_rbc(sr_ts *t, unsigned long long seed, double amp) {
+    RETCHK(need_scalar(t));
+    const long np = fsize(t->fp);
+    std::vector<double> T((size_t)np, 0.0);
+    uint64_t st = seed;
+    for (int i = 0; i < t->fp.n[0]; i++)
+        for (int j = 0; j < t->fp.n[1]; j++)
+            for (int k = 0; k < t->fp.n[2]; k++) {
+                st += 0x9E3779B97F4A7C15ull;
+                uint64_t z = st;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                double u = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+                T[(size_t)((((long)(i + 1) * (t->fp.n[1] + 2)) + (j + 1)) * (t->fp.n[2] + 2) + (k + 1))] =
+                    amp * (2.0 * u - 1.0);
+            }
+    HIPCHK(hipMemcpyAsync(t->T, T.data(), sizeof(double) * np, hipMemcpyHostToDevice, t->s));
+    ts_fill(t->P, np, 1.0, t->s);
+    ts_fill(t->Pp, np, 0.0, t->s);
+    for (int c = 0; c < 3; c++) ts_fill(t->V[c], fsize(t->fv[c]), 0.0, t->s);
+    for (int c = 0; c < 3; c++) impose_vbc(t, c);
+    impose_tbc(t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_impose_tbc(sr_ts *t) {
+    RETCHK(need_scalar(t));
+    impose_tbc(t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_build_trhs(sr_ts *t) {
+    RETCHK(need_scalar(t));
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    ts_build_trhs(t->fp, t->fv, t->M, Vc, t->T, t->Trhs, t->h, t->kappa, t->d.dt, t->s);
+    ts_ghost_wave(t->fp, t->Trhs, t->sync_p, t->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->s));
+    return SR_OK;
+}
+
+extern "C" int sr_ts_solve_t(sr_ts *t, int *iters) {
+    RETCHK(need_scalar(t));
+    return solve_t(t, iters);
+}
+
+extern "C" int sr_ts_nusselt(sr_ts *t, double *nu) {
+    RETCHK(need_scalar(t));
+    if (!nu) return set_err(SR_ERR_ARG, "null argument");
+    ts_nusselt(t->fp, t->fv[2], t->M, t->V[2], t->T, t->h, t->part, t->dred, t->s);
+    HIPCHK(hipMemcpyAsync(t->hred, t->dred, sizeof(double) * 2, hipMemcpyDeviceToHost, t->s));
+    HIPCHK(hipStreamSynchronize(t->s));
+    HIPCHK(hipGetLastError());
+    *nu = 1.0 + (t->hred[1] / t->hred[0]) / t->kappa;
     return SR_OK;
 }

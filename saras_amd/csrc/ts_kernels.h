@@ -21,11 +21,16 @@ struct Bulk {
     int lo[3], hi[3];
 };
 
-// scalars of the Jacobi viscous solve (hydro_d3.cc:375-410)
+// scalars of the Jacobi viscous / diffusive solves (hydro_d3.cc:375-410, scalar_d3.cc:378-410,
+// 538-570).  form 0: hydro_d3's 1/Re expressions; form 1: scalar_d3's nu (or kappa) expressions.
+// ms[d]: metric family of direction d (1 node, 0 face), read at the field's own indices.
 struct JacobiCoef {
     double hx, hy, hz;
     double hx2hy2, hy2hz2, hz2hx2, hx2hy2hz2;
     double dt, Re, inverseRe;
+    double vis;
+    int form;
+    int ms[3];
 };
 
 // one ghost / wall assignment F(dst) = a F(src) + b
@@ -49,8 +54,16 @@ struct GhostWave {
 
 void ts_deriv(const FShape &f, const Metrics &M, const double *F, double *out, int order, int d, double h,
               hipStream_t s);
+// nseRHS of component c; visc multiplies the diffusion (1/Re or nu); with T != null the
+// buoyancy Fb (T(k) + T(k+1)) / 2 is added to the z component
 void ts_build_rhs(int c, const FShape *fv, const FShape &fp, const Metrics &M, const double *const *V,
-                  const double *P, double *H, const double *h, double inverseRe, double dt, hipStream_t s);
+                  const double *P, double *H, const double *h, double visc, double dt, const double *T, double Fb,
+                  hipStream_t s);
+// tmpRHS of the temperature (scalar_d3.cc:349-357)
+void ts_build_trhs(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, const double *T,
+                   double *R, const double *h, double kappa, double dt, hipStream_t s);
+void ts_nusselt(const FShape &fp, const FShape &fz, const Metrics &M, const double *Vz, const double *T,
+                const double *h, double *part, double *out, hipStream_t s);
 void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
                const JacobiCoef &J, hipStream_t s);
 void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R,

@@ -66,7 +66,8 @@ __global__ void __launch_bounds__(256) k_build_rhs(int c, FShape fc, FShape fx, 
                                                    Metrics M, const double *__restrict__ Vx,
                                                    const double *__restrict__ Vy, const double *__restrict__ Vz,
                                                    const double *__restrict__ P, double *__restrict__ H, double h0,
-                                                   double h1, double h2, double inverseRe, double dt) {
+                                                   double h1, double h2, double inverseRe, double dt,
+                                                   const double *__restrict__ T, double Fb) {
     // one thread per padded entry: core entries get the right-hand side, pads a copy of V (the
     // reference forms nseRHS over the whole padded array, with zero terms outside the core)
     long total = (long)(fc.n[0] + 2) * (fc.n[1] + 2) * (fc.n[2] + 2);
@@ -128,9 +129,17 @@ __global__ void __launch_bounds__(256) k_build_rhs(int c, FShape fc, FShape fx, 
         double deriv = metv(M, d, fc.s[d], 1, idx[d]) * dv;
         acc -= inter * deriv / (double)ns;
     }
-    // - grad P on the faces of component c (sfield::gradient), * dt, + V
     long sp[3] = {(long)(fp.n[1] + 2) * (fp.n[2] + 2), (long)(fp.n[2] + 2), 1};
     long pq = fidx(fp, i, j, k);
+    // buoyantForce::addForcing: the z face between nodes k and k + 1 gets Fb times their mean
+    if (T && c == 2) {
+        double inter = 0.0;
+        inter += T[pq];
+        inter += T[pq + 1];
+        inter /= 2.0;
+        acc += Fb * inter;
+    }
+    // - grad P on the faces of component c (sfield::gradient), * dt, + V
     double g = metv(M, c, 0, 1, idx[c]) * (P[pq + sp[c]] - P[pq]) / hh[c];
     acc -= g;
     acc *= dt;
@@ -159,10 +168,14 @@ __global__ void __launch_bounds__(256) k_jacobi_v(FShape f, Metrics M, Bulk B, c
     }
     long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
     long q = t;
-    double mx = metv(M, 0, f.s[0], 3, i), my = metv(M, 1, f.s[1], 3, j), mz = metv(M, 2, f.s[2], 3, k);
-    G[q] = ((J.hy2hz2 * mx * (V[q + st[0]] + V[q - st[0]]) + J.hz2hx2 * my * (V[q + st[1]] + V[q - st[1]]) +
-             J.hx2hy2 * mz * (V[q + 1] + V[q - 1])) * J.dt / (J.hx2hy2hz2 * 2.0 * J.Re) + R[q]) /
-           (1.0 + J.dt * ((J.hy2hz2 * mx + J.hz2hx2 * my + J.hx2hy2 * mz)) / (J.Re * J.hx2hy2hz2));
+    double mx = metv(M, 0, J.ms[0], 3, i), my = metv(M, 1, J.ms[1], 3, j), mz = metv(M, 2, J.ms[2], 3, k);
+    double a = J.hy2hz2 * mx * (V[q + st[0]] + V[q - st[0]]) + J.hz2hx2 * my * (V[q + st[1]] + V[q - st[1]]) +
+               J.hx2hy2 * mz * (V[q + 1] + V[q - 1]);
+    double d = J.hy2hz2 * mx + J.hz2hx2 * my + J.hx2hy2 * mz;
+    if (J.form == 0)
+        G[q] = (a * J.dt / (J.hx2hy2hz2 * 2.0 * J.Re) + R[q]) / (1.0 + J.dt * (d) / (J.Re * J.hx2hy2hz2));
+    else
+        G[q] = (a * J.dt * J.vis / (J.hx2hy2hz2 * 2.0) + R[q]) / (1.0 + J.dt * J.vis * (d) / J.hx2hy2hz2);
 }
 
 // max over the bulk of |V - 0.5 dt/Re lap V - R|; maxima of non-negative doubles ordered as their
@@ -181,10 +194,11 @@ __global__ void __launch_bounds__(256) k_jacobi_res(FShape f, Metrics M, Bulk B,
         int i = (int)(r / sz[1]) + B.lo[0];
         long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
         long q = fidx(f, i, j, k);
-        double mx = metv(M, 0, f.s[0], 3, i), my = metv(M, 1, f.s[1], 3, j), mz = metv(M, 2, f.s[2], 3, k);
+        double mx = metv(M, 0, J.ms[0], 3, i), my = metv(M, 1, J.ms[1], 3, j), mz = metv(M, 2, J.ms[2], 3, k);
         double lap = V[q] - (mx * (V[q + st[0]] - 2.0 * V[q] + V[q - st[0]]) / (J.hx * J.hx) +
                              my * (V[q + st[1]] - 2.0 * V[q] + V[q - st[1]]) / (J.hy * J.hy) +
-                             mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (J.hz * J.hz)) * 0.5 * J.dt * J.inverseRe;
+                             mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (J.hz * J.hz)) * 0.5 * J.dt *
+                                (J.form == 0 ? J.inverseRe : J.vis);
         e = fabs(lap - R[q]);
     }
     __shared__ double s[4];
@@ -226,6 +240,92 @@ __global__ void __launch_bounds__(256) k_ghost_ops(FShape f, double *__restrict_
     }
     double v = op.a == 0.0 ? 0.0 : op.a * F[fidx(f, s[0], s[1], s[2])];
     F[fidx(f, p[0], p[1], p[2])] = v + op.b;
+}
+
+// ------------------------------------------------------------------------------------------------
+// temperature right-hand side: core nodes T + dt (kappa sum d2 T - sum_d Vbar_d dT/dx_d), pads T
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_build_trhs(FShape fp, FShape fx, FShape fy, FShape fz, Metrics M,
+                                                    const double *__restrict__ Vx, const double *__restrict__ Vy,
+                                                    const double *__restrict__ Vz, const double *__restrict__ T,
+                                                    double *__restrict__ R, double h0, double h1, double h2,
+                                                    double kappa, double dt) {
+    long total = (long)(fp.n[0] + 2) * (fp.n[1] + 2) * (fp.n[2] + 2);
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int k = (int)(t % (fp.n[2] + 2)) - 1;
+    long r = t / (fp.n[2] + 2);
+    int j = (int)(r % (fp.n[1] + 2)) - 1;
+    int i = (int)(r / (fp.n[1] + 2)) - 1;
+    if (i < 0 || j < 0 || k < 0 || i >= fp.n[0] || j >= fp.n[1] || k >= fp.n[2]) {
+        R[t] = T[t];
+        return;
+    }
+    long st[3] = {(long)(fp.n[1] + 2) * (fp.n[2] + 2), (long)(fp.n[2] + 2), 1};
+    int idx[3] = {i, j, k};
+    double hh[3] = {h0, h1, h2};
+    const double *V[3] = {Vx, Vy, Vz};
+    const FShape *FS[3] = {&fx, &fy, &fz};
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double ih = 1.0 / hh[d];
+        double ih2 = ih * ih;
+        double fpv = T[t + st[d]], fmv = T[t - st[d]], f0 = T[t];
+        double tmp = (fpv - fmv) / 2.0 * ih;
+        double sec = (fpv - 2.0 * f0 + fmv) * ih2;
+        acc += metv(M, d, 1, 2, idx[d]) * tmp + 0.5 * metv(M, d, 1, 3, idx[d]) * sec;
+    }
+    acc *= kappa;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int o[3] = {i, j, k};
+        o[d] -= 1;
+        double inter = 0.0;
+        inter += V[d][fidx(*FS[d], i, j, k)];
+        inter += V[d][fidx(*FS[d], o[0], o[1], o[2])];
+        double ih = 1.0 / hh[d];
+        double dv = (T[t + st[d]] - T[t - st[d]]) / 2.0;
+        dv *= ih;
+        double deriv = metv(M, d, 1, 1, idx[d]) * dv;
+        acc -= inter * deriv / 2.0;
+    }
+    acc *= dt;
+    acc += T[t];
+    R[t] = acc;
+}
+
+// Nusselt partials: per block sums of (volume) and (w T volume), w the mean of the two z faces
+__global__ void __launch_bounds__(256) k_nusselt(FShape fp, FShape fz, Metrics M, const double *__restrict__ Vz,
+                                                 const double *__restrict__ T, double h0, double h1, double h2,
+                                                 double *__restrict__ part) {
+    long total = (long)fp.n[0] * fp.n[1] * fp.n[2];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double vol = 0.0, uzt = 0.0;
+    if (t < total) {
+        int k = (int)(t % fp.n[2]);
+        long r = t / fp.n[2];
+        int j = (int)(r % fp.n[1]);
+        int i = (int)(r / fp.n[1]);
+        double dv = (h0 / metv(M, 0, 0, 1, i)) * (h1 / metv(M, 1, 0, 1, j)) * (h2 / metv(M, 2, 0, 1, k));
+        double w = (Vz[fidx(fz, i, j, k)] + Vz[fidx(fz, i, j, k - 1)]) / 2.0;
+        vol = dv;
+        uzt = w * T[fidx(fp, i, j, k)] * dv;
+    }
+    __shared__ double sv[4], su[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        vol += __shfl_xor(vol, o, 64);
+        uzt += __shfl_xor(uzt, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) { sv[threadIdx.x >> 6] = vol; su[threadIdx.x >> 6] = uzt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < 4; w++) { a += sv[w]; b += su[w]; }
+        part[2 * (long)blockIdx.x] = a;
+        part[2 * (long)blockIdx.x + 1] = b;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -403,11 +503,26 @@ void ts_deriv(const FShape &f, const Metrics &M, const double *F, double *out, i
 }
 
 void ts_build_rhs(int c, const FShape *fv, const FShape &fp, const Metrics &M, const double *const *V,
-                  const double *P, double *H, const double *h, double inverseRe, double dt, hipStream_t s) {
+                  const double *P, double *H, const double *h, double visc, double dt, const double *T, double Fb,
+                  hipStream_t s) {
     const FShape &fc = fv[c];
     long n = (long)(fc.n[0] + 2) * (fc.n[1] + 2) * (fc.n[2] + 2);
     hipLaunchKernelGGL(k_build_rhs, dim3(nb(n)), dim3(256), 0, s, c, fc, fv[0], fv[1], fv[2], fp, M, V[0], V[1],
-                       V[2], P, H, h[0], h[1], h[2], inverseRe, dt);
+                       V[2], P, H, h[0], h[1], h[2], visc, dt, T, Fb);
+}
+
+void ts_build_trhs(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, const double *T,
+                   double *R, const double *h, double kappa, double dt, hipStream_t s) {
+    long n = (long)(fp.n[0] + 2) * (fp.n[1] + 2) * (fp.n[2] + 2);
+    hipLaunchKernelGGL(k_build_trhs, dim3(nb(n)), dim3(256), 0, s, fp, fv[0], fv[1], fv[2], M, V[0], V[1], V[2], T, R,
+                       h[0], h[1], h[2], kappa, dt);
+}
+
+void ts_nusselt(const FShape &fp, const FShape &fz, const Metrics &M, const double *Vz, const double *T,
+                const double *h, double *part, double *out, hipStream_t s) {
+    int n = ts_energy_parts(fp);
+    hipLaunchKernelGGL(k_nusselt, dim3(n), dim3(256), 0, s, fp, fz, M, Vz, T, h[0], h[1], h[2], part);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, part, n, out);
 }
 
 void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,

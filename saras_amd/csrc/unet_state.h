@@ -51,6 +51,7 @@ struct sr_guess {
     double *rpart = nullptr, *dsc = nullptr, *hsc = nullptr;
     double kappa = 0.0;
     int trained = 0;
+    int policy = 0;   // 0: best of warm start / extrapolation / learned by residual; 1: learned always
     bool have_cache = false;
 };
 

@@ -114,3 +114,7 @@ class GuessNet:
     def extrapolate(self, p1_ptr, p2_ptr, out_ptr):
         _L.check(self.L.sr_guess_extrapolate(self.h, C.c_void_p(p1_ptr), C.c_void_p(p2_ptr) if p2_ptr else None,
                                              C.c_void_p(out_ptr)))
+
+    def set_policy(self, policy):
+        """0: best of warm start / extrapolation / learned by residual; 1: learned always."""
+        _L.check(self.L.sr_guess_set_policy(self.h, int(policy)))

@@ -87,3 +87,15 @@ def test_guess_create_rejects_bad_grid_without_device():
     ci, co, lv = C.c_int(), C.c_int(), C.c_int()
     assert lib.sr_guess_layer_info(4, C.byref(ci), C.byref(co), C.byref(lv)) == 0
     assert (ci.value, co.value, lv.value) == (32, 64, 2)
+
+
+def test_ts_scalar_descriptor_checks_without_device():
+    from saras_amd import _lib, ts
+    lib = _lib.load()
+    h = C.c_void_p()
+    d = ts.ts_desc(ind=(4, 4, 4), scalar=True, Ra=0.0)
+    assert lib.sr_ts_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    assert b"Ra" in lib.sr_last_error()
+    d = ts.ts_desc(ind=(4, 4, 4), scalar=True, rbc_type=9)
+    assert lib.sr_ts_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    assert b"rbcType" in lib.sr_last_error()
