@@ -111,6 +111,77 @@ def tgv_bench(args):
     print(json.dumps(out))
 
 
+def rbc_bench(args):
+    """Rayleigh-Benard convection (scalar_d3): pressure solves to 1e-10 relative residual inside the
+    time step.  The learned guess is trained online over the warm-up steps; then the same number of
+    steps is timed from the same state with (a) learned guess + mixed precision, (b) the reference's
+    warm start in FP64, (c) warm start in mixed precision."""
+    import numpy as np
+    from saras_amd import _lib, guess, ts
+    nz_ind = 8 if args.workload == "rbc257" else 7
+    kw = dict(ind=(8, 8, nz_ind), L=(1.0, 1.0, 1.0) if nz_ind == 8 else (2.0, 2.0, 1.0), domain="PPN",
+              mesh="UUD", beta=(1.0, 1.0, 1.0), scalar=True, rbc_type=2, Ra=1e7, Pr=1.0, dt=args.dt, prob_type=5,
+              Re=1.0, cn_tol=1e-6, semantics=_lib.SR_SEM_PROJECTED, rel_tol=1e-10, vc_count=100, pre=2, post=2,
+              res_type=2, depth=nz_ind - 1)
+    n = (257, 257, 257 if nz_ind == 8 else 129)
+    net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3)
+    run = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, **kw))
+    run.attach_guess(net, train=True)
+    run.initRBC(seed=0, amp=1e-3)
+    t0 = time.perf_counter()
+    train_loss = []
+    for _ in range(args.warmup):
+        st = run.timeAdvance()
+        if st.train_loss >= 0:
+            train_loss.append(st.train_loss)
+    train_s = time.perf_counter() - t0
+    snap = {w: run.download(w) for w in (ts.VX, ts.VY, ts.VZ, ts.P, ts.PP, ts.T)}
+    # (a) learned + mixed, frozen network, from the trained state
+    run.attach_guess(net, train=False)
+    rec_l = [run.timeAdvance() for _ in range(args.steps)]
+    learned = {"ms_per_solve": float(np.mean([r.solve_ms + r.guess_ms for r in rec_l])),
+               "cycles_mean": float(np.mean([r.mg_cycles for r in rec_l])),
+               "guess_ms_mean": float(np.mean([r.guess_ms for r in rec_l])),
+               "learned_chosen": int(sum(1 for r in rec_l if r.guess_choice == 2)),
+               "extrap_chosen": int(sum(1 for r in rec_l if r.guess_choice == 1)),
+               "rel_guess_median": float(np.median([r.rel_guess for r in rec_l])),
+               "step_ms_mean": float(np.mean([r.step_ms for r in rec_l]))}
+    nu_end = run.nusselt()
+    run.close()
+    out = {}
+    for name, prec in (("fp64_warm", _lib.SR_PREC_FP64), ("mixed_warm", _lib.SR_PREC_MIXED),
+                       ("gcr_warm", _lib.SR_PREC_MIXED_GCR)):
+        h = ts.Hydro(ts.ts_desc(precision=prec, **kw))
+        h.initRBC(seed=0, amp=1e-3)
+        for w, a in snap.items():
+            h.upload(w, a)
+        rec = [h.timeAdvance() for _ in range(args.steps)]
+        out[name] = {"ms_per_solve": float(np.mean([r.solve_ms for r in rec])),
+                     "cycles_mean": float(np.mean([r.mg_cycles for r in rec])),
+                     "step_ms_mean": float(np.mean([r.step_ms for r in rec]))}
+        h.close()
+    net.close()
+    base = out["fp64_warm"]["ms_per_solve"]
+    res = {
+        "metric": f"ms/pressure solve to 1e-10 rel. residual inside the RBC time step, {n[0]}x{n[1]}x{n[2]} "
+                  "(learned+mixed vs FP64 MG warm start)",
+        "value": round(learned["ms_per_solve"], 4), "unit": "ms/solve", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(learned["step_ms_mean"], 4), "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64+f32+bf16", "data": "synthetic",
+        "config": {"workload": f"Rayleigh-Benard {n[0]}x{n[1]}x{n[2]} nodes, PPN, tanh z beta=1, Ra=1e7, Pr=1, "
+                               f"rbcType 2, dt={args.dt}, T perturbation 1e-3 (seed 0); pressure solve V(2,2) to "
+                               "1e-10 rel. residual; guess trained online over the warm-up steps",
+                   "learned_mixed": learned, "fp64_warm": out["fp64_warm"], "mixed_warm": out["mixed_warm"],
+                   "gcr_warm": out["gcr_warm"],
+                   "mixed_solver": "FP32 V(2,2) cycle preconditioning flexible GCR(1) under the FP64 residual",
+                   "speedup_learned_mixed_vs_fp64_warm": round(base / learned["ms_per_solve"], 3),
+                   "train_seconds": round(train_s, 2), "train_loss_first": train_loss[:3],
+                   "train_loss_last": train_loss[-3:], "nusselt_end": nu_end},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,7 +189,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cycles", type=int, default=60)
-    ap.add_argument("--workload", default="poisson257", choices=["poisson257", "tgv129"])
+    ap.add_argument("--workload", default="poisson257", choices=["poisson257", "tgv129", "rbc257", "rbc257x129"])
+    ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the solver on one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +198,9 @@ def main():
         sys.exit(2)
     if args.workload == "tgv129":
         tgv_bench(args)
+        return
+    if args.workload.startswith("rbc"):
+        rbc_bench(args)
         return
 
     import numpy as np

@@ -38,6 +38,8 @@ extern "C" {
 
 #define SR_PREC_FP64  0
 #define SR_PREC_MIXED 1          /* FP32 inner V-cycles under an FP64 iterative-refinement loop   */
+#define SR_PREC_MIXED_GCR 2      /* FP32 V-cycle as the preconditioner of flexible GCR(1) with the
+                                    FP64 residual recomputed from the iterate every iteration      */
 
 typedef struct sr_mg sr_mg;
 
@@ -58,6 +60,10 @@ typedef struct sr_mg_desc {
                                     periodic x/y exchange); 1: pads follow each direction's ghost
                                     rule (mirror for Neumann, wrap for periodic z)                 */
     int coarseProject;           /* 1: project the coarsest RHS onto range(L) (singular only)    */
+    int coarseDirect;            /* 1: the coarsest level is solved exactly by a precomputed dense
+                                    inverse (bordered with the constant vector when L is singular,
+                                    which also projects the RHS) instead of Gauss-Seidel to
+                                    mgTolerance; at most 1024 coarsest nodes                        */
     double mgTolerance;          /* coarsest-solve absolute tolerance; stagnation tolerance      */
     double relTolerance;         /* SR_SEM_PROJECTED stop: ||f - L u||_2 / ||f||_2 < relTolerance */
     double dXi, dEt, dZt;        /* transformed-plane spacing, lib/grid/grid.cc:87-89           */
@@ -132,7 +138,8 @@ int sr_mg_kernel_bytes(const sr_mg *mg, int kernel, int level, double *bytes);
 
 /* Plan facts of a handle: what 0 = first level of the one-workgroup coarse cycle (FP64 hierarchy),
  * 1 = same for the FP32 hierarchy (a value above vcDepth means per-level launches throughout),
- * 2 / 3 = LDS bytes of those kernels, 4 = the device's LDS limit per workgroup.                   */
+ * 2 / 3 = LDS bytes of those kernels, 4 = the device's LDS limit per workgroup, 5 = iterations
+ * of the last coarsest-level solve (the count solve() reaches, poisson_d3.cc:190-247).            */
 int sr_mg_info(const sr_mg *mg, int what, long *value);
 
 /* ---- grid ------------------------------------------------------------------------------------ */

@@ -12,6 +12,7 @@
 #ifndef SARAS_TS_H
 #define SARAS_TS_H
 
+#include "saras_guess.h"
 #include "saras_mg.h"
 
 #ifdef __cplusplus
@@ -31,6 +32,8 @@ extern "C" {
 #define SR_TS_DIV 12
 #define SR_TS_T 13          /* temperature (scalar runs) */
 #define SR_TS_TRHS 14       /* its right-hand side tmpRHS */
+#define SR_TS_PP1 15        /* the previous two pressure corrections (kept with a guess attached) */
+#define SR_TS_PP2 16
 
 typedef struct sr_ts sr_ts;
 
@@ -50,12 +53,19 @@ typedef struct sr_ts_desc {
     double Ra, Pr;
     int semantics;                  /* pressure solve stop rule: SR_SEM_REFERENCE / _PROJECTED     */
     double relTolerance;
+    int precision;                  /* pressure solve: SR_PREC_FP64, _MIXED or _MIXED_GCR          */
+    int coarseDirect;               /* pressure solve: sr_mg_desc.coarseDirect                     */
 } sr_ts_desc;
 
 typedef struct sr_ts_stats {
     int jacobi_iters[4];            /* sweeps of solveVx / solveVy / solveVz / solveT              */
     int mg_cycles;                  /* V-cycles of the pressure solve                              */
     double step_ms;                 /* wall time of the call                                       */
+    double solve_ms;                /* wall time of the pressure solve (guess excluded)            */
+    double guess_ms;                /* wall time of sr_guess_predict (0 without a guess)           */
+    int guess_choice;               /* sr_guess_stats.choice, -1 without a guess                   */
+    double rel_guess;               /* relative residual of the guess handed to the solve          */
+    double train_loss;              /* sr_guess_train's relative loss, -1 when not training        */
 } sr_ts_stats;
 
 /* hydro_d3::hydro_d3 (src/solvers/hydro/hydro_d3.cc:63-119) without the initial condition */
@@ -88,6 +98,13 @@ int sr_ts_divergence(sr_ts *ts);
 /* tseries energy and maximum divergence (lib/io/tseries.cc:137-178) */
 int sr_ts_energy(sr_ts *ts, double *energy);
 int sr_ts_max_div(sr_ts *ts, double *maxdiv);
+
+/* With a guess handle attached (its solver grid = this run's node grid), each pressure solve
+ * starts from sr_guess_predict's guess built from the previous two corrections and the new
+ * right-hand side, and with train != 0 the network takes one training step on every converged
+ * correction.  g = NULL detaches (the reference's warm start from the previous correction).
+ * No counterpart in the reference.                                                               */
+int sr_ts_attach_guess(sr_ts *ts, sr_guess *g, int train);
 
 /* ---- scalar_d3 (src/solvers/scalar/scalar_d3.cc) ------------------------------------------- */
 /* the constructor's state (scalar_d3.cc:85-107: P = 1, T = 0, V = 0) with the temperature

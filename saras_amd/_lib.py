@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libsaras_mg.so")
 SR_OK, SR_ERR_ARG, SR_ERR_HIP, SR_ERR_STATE, SR_ERR_NOMEM, SR_ERR_NOT_CONVERGED = 0, 1, 2, 3, 4, 5
 SR_PRESSURE, SR_RESIDUAL, SR_TMP = 0, 1, 2
 SR_SEM_REFERENCE, SR_SEM_PROJECTED = 0, 1
-SR_PREC_FP64, SR_PREC_MIXED = 0, 1
+SR_PREC_FP64, SR_PREC_MIXED, SR_PREC_MIXED_GCR = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -24,7 +24,7 @@ class SrMgDesc(C.Structure):
         ("gsSmooth", C.c_int), ("resType", C.c_int),
         ("xPer", C.c_int), ("yPer", C.c_int), ("zPer", C.c_int),
         ("testPoisson", C.c_int), ("semantics", C.c_int), ("precision", C.c_int),
-        ("padMode", C.c_int), ("coarseProject", C.c_int),
+        ("padMode", C.c_int), ("coarseProject", C.c_int), ("coarseDirect", C.c_int),
         ("mgTolerance", C.c_double), ("relTolerance", C.c_double),
         ("dXi", C.c_double), ("dEt", C.c_double), ("dZt", C.c_double),
         ("xix2", _dp), ("xixx", _dp), ("ety2", _dp), ("etyy", _dp), ("ztz2", _dp), ("ztzz", _dp),
@@ -39,6 +39,7 @@ class SrMgStats(C.Structure):
 
 SR_TS_VX, SR_TS_VY, SR_TS_VZ, SR_TS_P, SR_TS_PP, SR_TS_MGRHS = 0, 1, 2, 3, 4, 5
 SR_TS_HX, SR_TS_HY, SR_TS_HZ, SR_TS_DIV, SR_TS_T, SR_TS_TRHS = 6, 7, 8, 12, 13, 14
+SR_TS_PP1, SR_TS_PP2 = 15, 16
 
 
 class SrTsDesc(C.Structure):
@@ -54,12 +55,15 @@ class SrTsDesc(C.Structure):
         ("resType", C.c_int), ("gsSmooth", C.c_int),
         ("mgTolerance", C.c_double),
         ("scalar", C.c_int), ("rbcType", C.c_int), ("Ra", C.c_double), ("Pr", C.c_double),
-        ("semantics", C.c_int), ("relTolerance", C.c_double),
+        ("semantics", C.c_int), ("relTolerance", C.c_double), ("precision", C.c_int),
+        ("coarseDirect", C.c_int),
     ]
 
 
 class SrTsStats(C.Structure):
-    _fields_ = [("jacobi_iters", C.c_int * 4), ("mg_cycles", C.c_int), ("step_ms", C.c_double)]
+    _fields_ = [("jacobi_iters", C.c_int * 4), ("mg_cycles", C.c_int), ("step_ms", C.c_double),
+                ("solve_ms", C.c_double), ("guess_ms", C.c_double), ("guess_choice", C.c_int),
+                ("rel_guess", C.c_double), ("train_loss", C.c_double)]
 
 
 SR_GUESS_LAYERS, SR_GUESS_IN_CH = 12, 8
@@ -127,6 +131,7 @@ SIGNATURES = [
     ("sr_ts_build_trhs", C.c_int, [_vp]),
     ("sr_ts_solve_t", C.c_int, [_vp, _ip]),
     ("sr_ts_nusselt", C.c_int, [_vp, _dp]),
+    ("sr_ts_attach_guess", C.c_int, [_vp, _vp, C.c_int]),
     ("sr_guess_create", C.c_int, [C.POINTER(SrGuessDesc), C.POINTER(C.c_void_p)]),
     ("sr_guess_destroy", C.c_int, [_vp]),
     ("sr_guess_layer_info", C.c_int, [C.c_int, _ip, _ip, _ip]),

@@ -118,12 +118,19 @@ __global__ void __launch_bounds__(256) k_sumsq_part(long n, const float *__restr
     if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x * 256];
 }
 
-__global__ void k_sum4_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
-    int c = threadIdx.x;
-    if (c >= 4) return;
+// the four sums over the block partials: 64 lanes per quantity, then a tree in LDS
+__global__ void __launch_bounds__(256) k_sum4_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+    __shared__ double red[256];
+    int c = threadIdx.x >> 6, l = threadIdx.x & 63;
     double s = 0.0;
-    for (int b = 0; b < nb; b++) s += part[b * 4 + c];
-    out[c] = s;
+    for (int b = l; b < nb; b += 64) s += part[b * 4 + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if (l < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (l == 0) out[c] = red[threadIdx.x];
 }
 
 __global__ void __launch_bounds__(256) k_make_input(long n, const float *__restrict__ st4, const double *__restrict__ ss,
@@ -288,7 +295,7 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
     hipLaunchKernelGGL(k_gather, dim3(nblk(n0.nvox())), dim3(256), 0, s, f, n0, g->ratio, g->rfield, p1, p2, rhs,
                        g->stage4);
     hipLaunchKernelGGL(k_sumsq_part, dim3(kRedBlocks), dim3(256), 0, s, n0.nvox(), g->stage4, g->rpart);
-    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(64), 0, s, g->rpart, kRedBlocks, g->dsc);
+    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(256), 0, s, g->rpart, kRedBlocks, g->dsc);
     hipLaunchKernelGGL(k_make_input, dim3(nblk(n0.nvox() * SR_GUESS_IN_CH)), dim3(256), 0, s, n0.nvox(), g->stage4,
                        g->dsc, g->T[X0]);
     RETCHK(guess_forward_device(g));
@@ -337,7 +344,7 @@ extern "C" int sr_guess_train(sr_guess *g, const double *p_star, double *rel_los
     // true correction on the network grid and its RMS: calibrates kappa
     hipLaunchKernelGGL(k_delta, dim3(kRedBlocks), dim3(256), 0, s, f, n0, g->ratio, p_star, g->extrap, g->y,
                        g->target, g->rpart);
-    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(64), 0, s, g->rpart, kRedBlocks, g->dsc + 4);
+    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(256), 0, s, g->rpart, kRedBlocks, g->dsc + 4);
     HIPCHK(hipMemcpyAsync(g->hsc, g->dsc, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const double dmean = g->hsc[5] / (double)n;
@@ -350,7 +357,7 @@ extern "C" int sr_guess_train(sr_guess *g, const double *p_star, double *rel_los
     if (g->kappa <= 0.0) return set_err(SR_ERR_STATE, "no residual or no correction to learn from");
     hipLaunchKernelGGL(k_loss, dim3(kRedBlocks), dim3(256), 0, s, n, g->target, g->y, g->dsc, g->dsc + 4, g->kappa,
                        g->dyd, g->rpart);
-    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(64), 0, s, g->rpart, kRedBlocks, g->dsc + 8);
+    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(256), 0, s, g->rpart, kRedBlocks, g->dsc + 8);
     RETCHK(guess_backward_device(g));
     RETCHK(guess_adam_device(g));
     HIPCHK(hipMemcpyAsync(g->hsc + 8, g->dsc + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, s));

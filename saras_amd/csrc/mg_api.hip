@@ -107,6 +107,90 @@ static void plan_coarse(sr_mg *m, int &first, CoarseDesc &D, size_t &lds, size_t
     }
 }
 
+// Dense inverse of the coarsest-level operator for coarseDirect.  A (nc x nc) is the 7-point
+// operator of the coarsest level with its face-ghost images folded in (periodic wrap, mirror,
+// antisymmetric image; walls homogeneous, as on every coarse level).  A singular A is bordered with
+// the constant vector, B = [[A, 1], [1^T, 0]]: for any f, B^{-1} [f; 0] = [u; lambda] with
+// A u = f - lambda 1 and sum u = 0, i.e. the solution of the RHS projected along the constant
+// vector (lambda = 0 for a consistent f).  The nc x nc block of B^{-1} is stored transposed.
+static bool coarse_singular(const sr_mg *m) { return !(m->d.testPoisson && !(m->d.xPer && m->d.yPer && m->d.zPer)); }
+
+static int build_coarse_direct(sr_mg *m) {
+    const Level &Cv = m->lv[m->depth];
+    const Lay &L = Cv.L;
+    int n[3] = {L.nx, L.ny, L.nz};
+    int nc = n[0] * n[1] * n[2];
+    if (nc > 1024) return set_err(SR_ERR_ARG, "coarseDirect needs at most 1024 coarsest-level nodes");
+    Bc bc = pressure_bc(m, m->depth, 1);
+    bool sing = coarse_singular(m);
+    int N = sing ? nc + 1 : nc;
+    std::vector<double> B((size_t)N * N, 0.0);
+    auto node = [&](const int *p) { return ((p[0] - 1) * n[1] + (p[1] - 1)) * n[2] + (p[2] - 1); };
+    for (int i = 1; i <= n[0]; i++)
+        for (int j = 1; j <= n[1]; j++)
+            for (int k = 1; k <= n[2]; k++) {
+                int p[3] = {i, j, k};
+                int row = node(p);
+                for (int d = 0; d < 3; d++) {
+                    const double *c = &Cv.coef_h[d][4 * (size_t)p[d]];
+                    B[(size_t)row * N + row] -= c[2];
+                    for (int sgn = -1; sgn <= 1; sgn += 2) {
+                        double w = sgn > 0 ? c[0] : c[1];
+                        int q[3] = {p[0], p[1], p[2]};
+                        q[d] += sgn;
+                        double mult = 1.0;
+                        if (q[d] < 1 || q[d] > n[d]) {
+                            int r = bc.rule[d];
+                            if (r != G_PERIODIC && r != G_MIRROR && r != G_ANTI) continue;   // homogeneous wall
+                            q[d] = ghost_src(r, q[d], n[d]);
+                            if (r == G_ANTI) mult = -1.0;
+                        }
+                        B[(size_t)row * N + node(q)] += mult * w;
+                    }
+                }
+                if (sing) {
+                    B[(size_t)row * N + nc] = 1.0;
+                    B[(size_t)nc * N + row] = 1.0;
+                }
+            }
+    // Gauss-Jordan inverse with partial pivoting
+    std::vector<double> I((size_t)N * N, 0.0);
+    for (int r = 0; r < N; r++) I[(size_t)r * N + r] = 1.0;
+    for (int col = 0; col < N; col++) {
+        int piv = col;
+        for (int r = col + 1; r < N; r++)
+            if (std::fabs(B[(size_t)r * N + col]) > std::fabs(B[(size_t)piv * N + col])) piv = r;
+        if (std::fabs(B[(size_t)piv * N + col]) < 1e-300) return set_err(SR_ERR_ARG, "coarsest operator is singular");
+        if (piv != col)
+            for (int c = 0; c < N; c++) {
+                std::swap(B[(size_t)piv * N + c], B[(size_t)col * N + c]);
+                std::swap(I[(size_t)piv * N + c], I[(size_t)col * N + c]);
+            }
+        double inv = 1.0 / B[(size_t)col * N + col];
+        for (int c = 0; c < N; c++) {
+            B[(size_t)col * N + c] *= inv;
+            I[(size_t)col * N + c] *= inv;
+        }
+        for (int r = 0; r < N; r++) {
+            if (r == col) continue;
+            double fct = B[(size_t)r * N + col];
+            if (fct == 0.0) continue;
+            for (int c = 0; c < N; c++) {
+                B[(size_t)r * N + c] -= fct * B[(size_t)col * N + c];
+                I[(size_t)r * N + c] -= fct * I[(size_t)col * N + c];
+            }
+        }
+    }
+    std::vector<double> pt((size_t)nc * nc);
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nc; j++) pt[(size_t)j * nc + i] = I[(size_t)i * N + j];
+    RETCHK(upload_vec(&m->pinv64, pt));
+    std::vector<float> pf(pt.begin(), pt.end());
+    RETCHK(upload_vec(&m->pinv32, pf));
+    m->ncd = nc;
+    return SR_OK;
+}
+
 static void finish_coarse(sr_mg *m, CoarseDesc &D, bool fp32) {
     for (int q = 0; q <= D.last - D.first; q++)
         for (int a = 0; a < 3; a++)
@@ -123,6 +207,8 @@ static void finish_coarse(sr_mg *m, CoarseDesc &D, bool fp32) {
     Level &Cv = m->lv[m->depth];
     for (int a = 0; a < 3; a++) D.wc[a] = Cv.wc[a];
     D.wcsum = Cv.wcsum;
+    D.pinv = fp32 ? (const void *)m->pinv32 : (const void *)m->pinv64;
+    D.ncd = m->ncd;
 }
 
 static int create_impl(const sr_mg_desc *d, sr_mg *m) {
@@ -194,7 +280,18 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
             RETCHK(upload_vec(&m->wall[q], w));
         }
     }
-    if (d->precision == SR_PREC_MIXED) RETCHK(ensure_fp32(m));
+    if (d->precision != SR_PREC_FP64 && d->precision != SR_PREC_MIXED && d->precision != SR_PREC_MIXED_GCR)
+        return set_err(SR_ERR_ARG, "unknown precision mode");
+    if (d->precision != SR_PREC_FP64) RETCHK(ensure_fp32(m));
+    if (d->precision == SR_PREC_MIXED_GCR) {
+        long sz = lay_size(F);
+        for (int q = 0; q < 2; q++) HIPCHK(hipMalloc((void **)&m->zdir[q], sizeof(float) * sz));
+        HIPCHK(hipMalloc((void **)&m->gpart, sizeof(double) * 5 * (size_t)norm_res_parts(F)));
+        HIPCHK(hipMalloc((void **)&m->gcoef, sizeof(double) * 8));
+        // wall ghosts of the directions are never written: they stay zero like the correction's
+        for (int q = 0; q < 2; q++) launch_fill<float>(m->zdir[q], sz, 0.0f, m->stream);
+    }
+    if (d->coarseDirect) RETCHK(build_coarse_direct(m));
     // coarse end of the cycle in LDS
     {
         int dev = 0, lds_max = 65536;
@@ -250,6 +347,9 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
 
     int nparts = norm_parts(F);
     HIPCHK(hipMalloc((void **)&m->part, sizeof(double) * 4 * (size_t)nparts));
+    HIPCHK(hipMalloc((void **)&m->part2, sizeof(double) * 2 * (size_t)norm_res_parts(F)));
+    HIPCHK(hipMalloc((void **)&m->fterms, sizeof(double) * 4));
+    launch_fill<double>(m->fterms, 4, 0.0, m->stream);
     HIPCHK(hipMalloc((void **)&m->dres, sizeof(double) * 16));
     HIPCHK(hipMalloc((void **)&m->diters, sizeof(int) * 4));
     HIPCHK(hipHostMalloc((void **)&m->hres, sizeof(double) * 16, hipHostMallocDefault));
@@ -280,7 +380,12 @@ extern "C" int sr_mg_destroy(sr_mg *m) {
     for (int q = 0; q < 2; q++) {
         if (m->graph_mixed[q]) (void)hipGraphExecDestroy(m->graph_mixed[q]);
         if (m->graph_mixed_g[q]) (void)hipGraphDestroy(m->graph_mixed_g[q]);
+        if (m->graph_gcr[q]) (void)hipGraphExecDestroy(m->graph_gcr[q]);
+        if (m->graph_gcr_g[q]) (void)hipGraphDestroy(m->graph_gcr_g[q]);
+        (void)hipFree(m->zdir[q]);
     }
+    (void)hipFree(m->pinv64); (void)hipFree(m->pinv32);
+    (void)hipFree(m->gpart); (void)hipFree(m->gcoef);
     for (Level &V : m->lv) {
         (void)hipFree(V.u); (void)hipFree(V.f); (void)hipFree(V.t);
         (void)hipFree(V.u32); (void)hipFree(V.f32); (void)hipFree(V.t32);
@@ -288,7 +393,7 @@ extern "C" int sr_mg_destroy(sr_mg *m) {
     }
     for (int q = 0; q < 6; q++) (void)hipFree(m->wall[q]);
     for (int a = 0; a < 3; a++) (void)hipFree(m->wvec[a]);
-    (void)hipFree(m->part); (void)hipFree(m->dres); (void)hipFree(m->diters); (void)hipFree(m->stage);
+    (void)hipFree(m->part); (void)hipFree(m->part2); (void)hipFree(m->fterms); (void)hipFree(m->dres); (void)hipFree(m->diters); (void)hipFree(m->stage);
     (void)hipFree(m->stage2); (void)hipFree(m->r32);
     if (m->hres) (void)hipHostFree(m->hres);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -398,6 +503,11 @@ void sr::residual_level(sr_mg *m, int l) {
 void sr::coarsest_level(sr_mg *m, int l, int zeroBC) {
     Level &V = m->lv[l];
     Bc bc = pressure_bc(m, l, zeroBC);
+    if (m->pinv64 && l == m->depth) {
+        for (int q = 0; q < 6; q++) bc.wall[q] = nullptr;
+        launch_coarsest_direct<double>(V.L, V.u, V.f, m->pinv64, m->ncd, bc, m->diters, m->stream);
+        return;
+    }
     launch_coarsest<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], bc, m->d.mgTolerance, m->maxCount,
                             m->diters, m->stream);
 }
@@ -441,7 +551,7 @@ void sr::vcycle_fp64(sr_mg *m) {
 // FP32 hierarchy (mixed precision): the V-cycle on the error equation A e = r with e = 0 at start.
 // Level-0 ghosts follow the BCs of the error: periodic / mirror as for u, Dirichlet walls -> 0.
 // ------------------------------------------------------------------------------------------------
-static Bc error_bc(const sr_mg *m, int l, int zeroBC) {
+Bc sr::error_bc(const sr_mg *m, int l, int zeroBC) {
     Bc b = pressure_bc(m, l, zeroBC);
     for (int q = 0; q < 6; q++) b.wall[q] = nullptr;
     return b;
@@ -482,6 +592,10 @@ void sr::vcycle_fp32(sr_mg *m) {
             launch_coarse_cycle<float>(m->cdesc32, m->clds32, m->lv[Lc].u32, m->lv[Lc].f32, m->diters, s);
         } else if (l + 1 == D) {
             Level &Cv = m->lv[D];
+            if (m->pinv32) {
+                launch_coarsest_direct<float>(Cv.L, Cv.u32, Cv.f32, m->pinv32, m->ncd, error_bc(m, D, 1), m->diters, s);
+                continue;
+            }
             launch_fill<float>(Cv.u32, lay_size(Cv.L), 0.0f, s);
             if (d.coarseProject && singular(m))
                 launch_project_small<float>(Cv.L, Cv.f32, Cv.wc[0], Cv.wc[1], Cv.wc[2], Cv.wcsum, s);
@@ -514,7 +628,12 @@ int sr::ensure_fp32(sr_mg *m) {
 
 void sr::norm_level0(sr_mg *m) {
     Level &V = m->lv[0];
-    launch_norm<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part, m->dres, m->stream);
+    launch_norm_res<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part2, m->fterms, m->dres, m->stream);
+}
+
+void sr::f_terms_level0(sr_mg *m) {
+    Level &V = m->lv[0];
+    launch_f_terms<double>(V.L, V.f, m->part, m->fterms, m->stream);
 }
 
 // computeError from the four reduced quantities (poisson_d3.cc:433-458)
@@ -660,6 +779,7 @@ extern "C" int sr_mg_op_update_pads(sr_mg *m, int which) {
 extern "C" int sr_mg_op_compute_error(sr_mg *m, int norm, double *out) {
     if (!m || !out) return set_err(SR_ERR_ARG, "null");
     if (m->vLevel != 0) return set_err(SR_ERR_STATE, "computeError runs on the finest level");
+    f_terms_level0(m);
     norm_level0(m);
     HIPCHK(hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream));
     RETCHK(sync_check(m));
@@ -683,6 +803,12 @@ extern "C" int sr_mg_info(const sr_mg *m, int what, long *value) {
         case 2: *value = (long)m->clds64; break;
         case 3: *value = (long)m->clds32; break;
         case 4: *value = m->lds_limit; break;
+        case 5: {
+            int it = 0;
+            HIPCHK(hipMemcpy(&it, m->diters, sizeof(int), hipMemcpyDeviceToHost));
+            *value = it;
+            break;
+        }
         default: return set_err(SR_ERR_ARG, "unknown info key");
     }
     return SR_OK;

@@ -24,20 +24,40 @@ void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int c
 template <typename TS, typename TD> void launch_pack(const Lay &L, const TS *ref, TD *dst, int core_only, hipStream_t s);
 template <typename TS, typename TD> void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s);
 int norm_parts(const Lay &L);
-// out[0..3] = sum |Lu-f|^2, sum f^2 (core), max |Lu-f| (core), max |f| (every padded node)
-template <typename T>
-void launch_norm(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part,
-                 double *out, hipStream_t s);
 // out[0] = sum_core w_i w_j w_k f, out[1] = sum over every padded node of f
 template <typename T>
 void launch_rhs_sums(const Lay &L, const T *f, const double *wx, const double *wy, const double *wz, double *part,
                      double *out, hipStream_t s);
 template <typename T> size_t coarsest_lds_bytes(const Lay &L);
+// blocks of the one-node-per-thread residual norms (two partials each)
+int norm_res_parts(const Lay &L);
+// fterms[1] = sum f^2 (core), fterms[3] = max |f| (every padded node); the right-hand side part of
+// computeError, reduced once per right-hand side
+template <typename T> void launch_f_terms(const Lay &L, const T *f, double *part, double *fterms, hipStream_t s);
+// out[0..3] = sum |Lu-f|^2, sum f^2 (core), max |Lu-f| (core), max |f| (every padded node);
+// out[1] and out[3] are copied from fterms
+template <typename T>
+void launch_norm_res(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part2,
+                     const double *fterms, double *out, hipStream_t s);
 // iterative-refinement step: e == nullptr computes r = f - L u_in only; otherwise u_out = u_in + e
-// (core and face ghosts) and r = f - L u_out.  r is stored in FP32; out[0..3] as launch_norm.
+// (core and face ghosts) and r = f - L u_out.  r is stored in FP32; out[0..3] as launch_norm_res.
 void launch_outer_residual(const Lay &L, const double *u_in, const float *e, const double *f, double *u_out,
                            float *r32, const double *cx, const double *cy, const double *cz, const Bc &bc,
-                           double *part, double *out, hipStream_t s);
+                           double *part2, const double *fterms, double *out, hipStream_t s);
+// Krylov-accelerated mixed precision (flexible GCR(1), FP32 V-cycle preconditioner): the step
+// coefficients from e = M r32 and the previous direction z (coef[0] alpha, coef[1] beta; part5 holds
+// 5 doubles per block of norm_res_parts), then the update x_out = x_in + alpha (e - beta z),
+// z_next = e - beta z and the FP64 residual r = f - L x_out (stored FP32; out[0..3] as launch_norm_res)
+void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const double *cx,
+                     const double *cy, const double *cz, double *part5, double *coef, hipStream_t s);
+void launch_gcr_update(const Lay &L, const double *x_in, const float *e, const float *z, const double *f,
+                       const double *coef, double *x_out, float *z_next, float *r32, const double *cx,
+                       const double *cy, const double *cz, const Bc &bc, const Bc &ebc, double *part2,
+                       const double *fterms, double *out, hipStream_t s);
+// exact coarsest solve u = P f with the dense inverse (transposed, nc x nc), ghosts filled by bc
+template <typename T>
+void launch_coarsest_direct(const Lay &L, T *u, const T *f, const T *pt, int nc, const Bc &bc, int *iters,
+                            hipStream_t s);
 template <typename T>
 void launch_coarsest(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, const Bc &bc,
                      double tol, int max_count, int *iters, hipStream_t s);

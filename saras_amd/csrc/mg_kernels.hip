@@ -406,13 +406,98 @@ __device__ inline void block_reduce4(double s0, double s1, double m2, double m3,
     }
 }
 
-// level-0 residual norms: q0 = sum |Lu - f|^2, q1 = sum f^2 (core), q2 = max |Lu - f|,
-// q3 = max |f| over every padded node (computeError(1) denominator, poisson_d3.cc:437)
+// per-block (sum, max) of one value per thread, written at the hardware block index
+__device__ inline void block_reduce2(double s, double m, double *part) {
+    __shared__ double rs[4], rm[4];
+    s = wave_sum(s);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) {
+        rs[threadIdx.x >> 6] = s;
+        rm[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+            a += rs[w];
+            b = fmax(b, rm[w]);
+        }
+        part[2 * (long)blockIdx.x] = a;
+        part[2 * (long)blockIdx.x + 1] = b;
+    }
+}
+
+// Reduction kernels visit the core nodes of four consecutive x-planes per thread (logical planes
+// are plane groups: geo_core4), so one block reduction serves up to 1024 nodes.
+template <typename F>
+__device__ inline void core4_visit(const Lay &L, const Geo &G, F &&fn) {
+    unsigned gu, rr;
+    if (!geo_slot(G, xcd_block(), gu, rr)) return;
+    unsigned jr = fdivu(rr, G.d1, G.inv1);
+    unsigned r2 = rr - jr * G.d1;
+    int jp = 1 + (int)jr;
+    int c = r2 >= (unsigned)L.kh ? 1 : 0;
+    int h = (int)r2 - c * L.kh;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        int ip = 1 + 4 * (int)gu + q;
+        int kb = (c + 1 + ip + jp) & 1;
+        int kp = 2 * h + kb;
+        if (ip <= L.nx && kp >= 1 && kp <= L.nz) fn(ip, jp, c, h, kb, kp);
+    }
+}
+
+// per-cycle residual norm: sum |Lu - f|^2 and max |Lu - f| per block.  The right-hand side terms of computeError do not change during a solve
+// and are reduced once per solve (k_f_terms).
 template <typename T>
-__global__ void __launch_bounds__(256) k_norm_partial(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
-                                                      const T *__restrict__ cx, const T *__restrict__ cy,
-                                                      const T *__restrict__ cz, double *__restrict__ part) {
-    double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
+__global__ void __launch_bounds__(256) k_norm_res(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
+                                                  const T *__restrict__ cx, const T *__restrict__ cy,
+                                                  const T *__restrict__ cz, double *__restrict__ part) {
+    double s = 0.0, mm = 0.0;
+    core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
+        long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+        double lu = (double)apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+        double v = fabs(lu - (double)f[rc + h]);
+        s += v * v;
+        mm = fmax(mm, v);
+    });
+    block_reduce2(s, mm, part);
+}
+
+// the four computeError quantities from the per-block residual partials and the once-per-solve
+// right-hand side terms ft = {-, sum f^2 (core), -, max |f| (every padded node)}
+__global__ void __launch_bounds__(1024) k_norm_combine(const double *__restrict__ part, int nb,
+                                                       const double *__restrict__ ft, double *__restrict__ out) {
+    double s = 0.0, m = 0.0;
+    for (int b = (int)threadIdx.x; b < nb; b += 1024) {
+        s += part[2 * (long)b];
+        m = fmax(m, part[2 * (long)b + 1]);
+    }
+    __shared__ double rs[16], rm[16];
+    s = wave_sum(s);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) {
+        rs[threadIdx.x >> 6] = s;
+        rm[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int w = 0; w < 16; ++w) {
+            a += rs[w];
+            b = fmax(b, rm[w]);
+        }
+        out[0] = a;
+        out[1] = ft[1];
+        out[2] = b;
+        out[3] = ft[3];
+    }
+}
+
+// sum f^2 over the core and max |f| over every padded node
+template <typename T>
+__global__ void __launch_bounds__(256) k_f_terms(Lay L, Geo G, const T *__restrict__ f, double *__restrict__ part) {
+    double s1 = 0, m3 = 0;
     unsigned lb0, lbs, lbe;
     xcd_range(G.total_blocks, lb0, lbs, lbe);
     for (unsigned lb = lb0; lb < lbe; lb += lbs) {
@@ -423,22 +508,13 @@ __global__ void __launch_bounds__(256) k_norm_partial(Lay L, Geo G, const T *__r
         int c = r2 >= (unsigned)L.khs ? 1 : 0;
         int h = (int)r2 - c * L.khs;
         int ip = (int)ipu, jp = (int)jpu;
-        int kb = (c + 1 + ip + jp) & 1;
-        int kp = 2 * h + kb;
+        int kp = 2 * h + ((c + 1 + ip + jp) & 1);
         if (h >= L.kh || kp > L.nz + 1) continue;
-        long t = (long)ip * G.per_plane + r;
-        double fv = (double)f[t];
+        double fv = (double)f[(long)ip * G.per_plane + r];
         m3 = fmax(m3, fabs(fv));
-        if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
-            long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
-            double lu = (double)apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
-            double v = fabs(lu - fv);
-            s0 += v * v;
-            s1 += fv * fv;
-            m2 = fmax(m2, v);
-        }
+        if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) s1 += fv * fv;
     }
-    block_reduce4(s0, s1, m2, m3, part);
+    block_reduce4(0.0, s1, 0.0, m3, part);
 }
 
 // weighted sums over the reference right-hand side: q0 = sum w_i w_j w_k f (core), q1 = sum f over
@@ -634,44 +710,143 @@ __global__ void __launch_bounds__(256) k_outer_residual(Lay L, Geo G, const doub
                                                         const double *__restrict__ cx, const double *__restrict__ cy,
                                                         const double *__restrict__ cz, Bc bc,
                                                         double *__restrict__ part) {
-    double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
+    double s = 0.0, mm = 0.0;
     const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
-    unsigned lb0, lbs, lbe;
-    xcd_range(G.total_blocks, lb0, lbs, lbe);
-    for (unsigned lb = lb0; lb < lbe; lb += lbs) {
-        unsigned ipu, r;
-        if (!geo_slot(G, lb, ipu, r)) continue;
-        unsigned jpu = fdivu(r, G.d1, G.inv1);
-        unsigned r2 = r - jpu * G.d1;
-        int c = r2 >= (unsigned)L.khs ? 1 : 0;
-        int h = (int)r2 - c * L.khs;
-        int ip = (int)ipu, jp = (int)jpu;
-        int kb = (c + 1 + ip + jp) & 1;
-        int kp = 2 * h + kb;
-        if (h >= L.kh || kp > L.nz + 1) continue;
-        long t = (long)ip * G.per_plane + r;
-        double fv = f[t];
-        m3 = fmax(m3, fabs(fv));
-        if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
-            long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
-            auto U = [&](long q) { return CORR ? u_in[q] + (double)e[q] : u_in[q]; };
-            double uc = U(rc + h);
-            const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
-            double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) +
-                        Y[1] * U(ro - sy + h) + Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) -
-                        (X[2] + Y[2] + Z[2]) * uc;
-            double rv = fv - lu;
-            r32[t] = (float)rv;
-            if (CORR) {
-                u_out[t] = uc;
-                write_images(L, u_out, ip, jp, kp, uc, bc);
-            }
-            s0 += rv * rv;
-            s1 += fv * fv;
-            m2 = fmax(m2, fabs(rv));
+    core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
+        long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+        auto U = [&](long q) { return CORR ? u_in[q] + (double)e[q] : u_in[q]; };
+        double uc = U(rc + h);
+        const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) + Y[1] * U(ro - sy + h) +
+                    Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) - (X[2] + Y[2] + Z[2]) * uc;
+        double rv = f[rc + h] - lu;
+        r32[rc + h] = (float)rv;
+        if (CORR) {
+            u_out[rc + h] = uc;
+            write_images(L, u_out, ip, jp, kp, uc, bc);
         }
+        s += rv * rv;
+        mm = fmax(mm, fabs(rv));
+    });
+    block_reduce2(s, mm, part);
+}
+
+// ------------------------------------------------------------------------------------------------
+// mixed precision, Krylov-accelerated: flexible GCR(1) preconditioned by one FP32 V-cycle, with the
+// FP64 residual recomputed from the iterate every iteration (no recursive residual drift).
+//   e      FP32 V-cycle result for the right-hand side r32 (the preconditioned residual)
+//   z      previous search direction (FP32; with zero it reduces to a minimal-residual step)
+//   q = L e, p = L z;  beta = (q, p) / (p, p);  d = e - beta z;  alpha = (r, L d) / (L d, L d)
+//   x_out = x_in + alpha d,  r = f - L x_out
+// The directions and the inner products use FP32 data (they only steer the step); the iterate and
+// its residual are FP64, so the attained accuracy is that of the FP64 residual.
+// ------------------------------------------------------------------------------------------------
+template <typename TA>
+__device__ inline double lap_f(const Lay &L, const TA *__restrict__ a, long rc, long ro, int h, int kb,
+                               const double *X, const double *Y, const double *Z) {
+    const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
+    return X[0] * (double)a[ro + sx + h] + X[1] * (double)a[ro - sx + h] + Y[0] * (double)a[ro + sy + h] +
+           Y[1] * (double)a[ro - sy + h] + Z[0] * (double)a[ro + h + kb] + Z[1] * (double)a[ro + h - 1 + kb] -
+           (X[2] + Y[2] + Z[2]) * (double)a[rc + h];
+}
+
+// per-block sums (r q, q q, q p, r p, p p), written at the hardware block index
+__global__ void __launch_bounds__(256) k_gcr_dots(Lay L, Geo G, const float *__restrict__ e,
+                                                  const float *__restrict__ z, const float *__restrict__ r32,
+                                                  const double *__restrict__ cx, const double *__restrict__ cy,
+                                                  const double *__restrict__ cz, double *__restrict__ part) {
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
+        long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+        const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        double q = lap_f<float>(L, e, rc, ro, h, kb, X, Y, Z);
+        double p = lap_f<float>(L, z, rc, ro, h, kb, X, Y, Z);
+        double r = (double)r32[rc + h];
+        s[0] += r * q;
+        s[1] += q * q;
+        s[2] += q * p;
+        s[3] += r * p;
+        s[4] += p * p;
+    });
+    __shared__ double sh[4][5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) s[i] = wave_sum(s[i]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 5; i++) sh[threadIdx.x >> 6][i] = s[i];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double a = 0.0;
+        for (int w = 0; w < 4; w++) a += sh[w][threadIdx.x];
+        part[5 * (long)blockIdx.x + threadIdx.x] = a;
     }
-    block_reduce4(s0, s1, m2, m3, part);
+}
+
+// alpha, beta of the step from the block sums; coef = {alpha, beta, (r q), (q q), (q p), (r p), (p p)}.
+// When the orthogonalised direction has lost most of its norm to cancellation (or there is no
+// previous direction) the step falls back to the minimal-residual step along e.
+__global__ void __launch_bounds__(1024) k_gcr_coef(const double *__restrict__ part, int nb, double *__restrict__ coef) {
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = (int)threadIdx.x; b < nb; b += 1024)
+#pragma unroll
+        for (int i = 0; i < 5; i++) s[i] += part[5 * (long)b + i];
+    __shared__ double sh[16][5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) s[i] = wave_sum(s[i]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 5; i++) sh[threadIdx.x >> 6][i] = s[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < 16; w++)
+            for (int i = 0; i < 5; i++) t[i] += sh[w][i];
+        double rq = t[0], qq = t[1], qp = t[2], rp = t[3], pp = t[4];
+        double beta = pp > 0.0 ? qp / pp : 0.0;
+        double qn = qq - 2.0 * beta * qp + beta * beta * pp;
+        if (!(qn >= 1e-3 * qq)) {
+            beta = 0.0;
+            qn = qq;
+        }
+        double alpha = qn > 0.0 ? (rq - beta * rp) / qn : 0.0;
+        coef[0] = alpha;
+        coef[1] = beta;
+        for (int i = 0; i < 5; i++) coef[2 + i] = t[i];
+    }
+}
+
+// x_out = x_in + alpha (e - beta z) on core nodes (face-ghost images by bc), z_next = e - beta z
+// (images by ebc), r = f - L x_out in FP64 stored as r32, block partials (sum r^2, max |r|)
+__global__ void __launch_bounds__(256) k_gcr_update(Lay L, Geo G, const double *__restrict__ x_in,
+                                                    const float *__restrict__ e, const float *__restrict__ z,
+                                                    const double *__restrict__ f, const double *__restrict__ coef,
+                                                    double *__restrict__ x_out, float *__restrict__ z_next,
+                                                    float *__restrict__ r32, const double *__restrict__ cx,
+                                                    const double *__restrict__ cy, const double *__restrict__ cz,
+                                                    Bc bc, Bc ebc, double *__restrict__ part) {
+    double s = 0.0, mm = 0.0;
+    const double alpha = coef[0], beta = coef[1];
+    const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
+    core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
+        long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+        auto D = [&](long q) { return (double)e[q] - beta * (double)z[q]; };
+        auto U = [&](long q) { return x_in[q] + alpha * D(q); };
+        double dc = D(rc + h);
+        double uc = x_in[rc + h] + alpha * dc;
+        const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) + Y[1] * U(ro - sy + h) +
+                    Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) - (X[2] + Y[2] + Z[2]) * uc;
+        double rv = f[rc + h] - lu;
+        r32[rc + h] = (float)rv;
+        x_out[rc + h] = uc;
+        write_images(L, x_out, ip, jp, kp, uc, bc);
+        float dz = (float)dc;
+        z_next[rc + h] = dz;
+        write_images(L, z_next, ip, jp, kp, dz, ebc);
+        s += rv * rv;
+        mm = fmax(mm, fabs(rv));
+    });
+    block_reduce2(s, mm, part);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -835,6 +1010,76 @@ __device__ inline void blk_project(const Lay &L, T *f, const double *wx, const d
     __syncthreads();
 }
 
+// The coarsest-level iteration of the coarse cycle in one wavefront: the coarsest level has a few
+// dozen nodes, so one wave updates them with wave-level synchronisation instead of workgroup
+// barriers.  Same red-black order, same ghost composition, same stop rule as the whole-block path.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T>
+__device__ inline void wv_fill_ghosts(const Lay &L, T *a, const Bc &bc, int lane) {
+    int NX = L.nx + 2, NY = L.ny + 2, NZ = L.nz + 2;
+    int n = NX * NY * NZ;
+    for (int t = lane; t < n; t += 64) {
+        int kp = t % NZ, r = t / NZ, jp = r % NY, ip = r / NY;
+        bool ghost = ip == 0 || ip == NX - 1 || jp == 0 || jp == NY - 1 || kp == 0 || kp == NZ - 1;
+        T v;
+        if (ghost && ghost_resolve(L, a, ip, jp, kp, bc, v)) a[lay_idx(L, ip, jp, kp)] = v;
+    }
+    wave_sync();
+}
+
+template <typename T>
+__device__ inline void wv_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
+                                int lane) {
+    int n = L.nx * L.ny * L.nz;
+    for (int t = lane; t < n; t += 64) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        if (((i + j + k) & 1) != color) continue;
+        int ip = i + 1, jp = j + 1, kp = k + 1;
+        const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        T num = X[0] * u[lay_idx(L, ip + 1, jp, kp)] + X[1] * u[lay_idx(L, ip - 1, jp, kp)] +
+                Y[0] * u[lay_idx(L, ip, jp + 1, kp)] + Y[1] * u[lay_idx(L, ip, jp - 1, kp)] +
+                Z[0] * u[lay_idx(L, ip, jp, kp + 1)] + Z[1] * u[lay_idx(L, ip, jp, kp - 1)] - f[lay_idx(L, ip, jp, kp)];
+        u[lay_idx(L, ip, jp, kp)] = num / (X[2] + Y[2] + Z[2]);
+    }
+    wave_sync();
+}
+
+template <typename T>
+__device__ inline double wv_max_res(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz,
+                                    int lane) {
+    double m = -1.0e-10;
+    int nc = L.nx * L.ny * L.nz;
+    for (int t = lane; t < nc; t += 64) {
+        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int ip = i + 1, jp = j + 1, kp = k + 1;
+        int c = lay_color(ip, jp, kp);
+        long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
+        double lu = (double)apply_lap<T, T>(L, u, rc, ro, kp >> 1, kp & 1, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
+        m = fmax(m, fabs((double)f[rc + (kp >> 1)] - lu));
+    }
+    return wave_max(m);
+}
+
+// u = P f on the core nodes of the coarsest level (P transposed: pt[j * nc + i] = P[i][j]), node
+// order (i, j, k) with k fastest; ghosts are left to the caller.  FP64 accumulation.
+template <typename T>
+__device__ inline void blk_direct(const Lay &L, T *u, const T *f, const T *__restrict__ pt, int nc) {
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        double acc = 0.0;
+        int j = 0;
+        for (int a = 1; a <= L.nx; a++)
+            for (int b = 1; b <= L.ny; b++)
+                for (int c = 1; c <= L.nz; c++, j++) acc += (double)pt[(long)j * nc + i] * (double)f[lay_idx(L, a, b, c)];
+        int k = i % L.nz, r = i / L.nz;
+        u[lay_idx(L, 1 + r / L.ny, 1 + r % L.ny, 1 + k)] = (T)acc;
+    }
+    __syncthreads();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restrict__ u_first, const T *__restrict__ f_first,
                                                        int *__restrict__ iters) {
@@ -880,22 +1125,30 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
         int q = nl - 1;
         const Lay &L = D.L[q];
         blk_zero<T>(L, UU(q));
+        if (D.pinv) {
+            blk_direct<T>(L, UU(q), FF(q), (const T *)D.pinv, D.ncd);
+            lds_fill_ghosts(L, UU(q), D.bc);
+            if (threadIdx.x == 0) iters[0] = 0;
+        } else {
         if (D.project) blk_project<T>(L, FF(q), D.wc[0], D.wc[1], D.wc[2], D.wcsum, red);
-        int it = 0;
-        while (true) {
-            lds_fill_ghosts(L, UU(q), D.bc);
-            lds_sweep(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0);
-            __syncthreads();
-            lds_fill_ghosts(L, UU(q), D.bc);
-            lds_sweep(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 1);
-            __syncthreads();
-            double gm = blk_max_res<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), red);
-            if (gm < D.tol) break;
-            it += 1;
-            if (it > D.max_count) break;
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            int it = 0;
+            while (true) {
+                wv_fill_ghosts(L, UU(q), D.bc, lane);
+                wv_sweep(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0, lane);
+                wv_fill_ghosts(L, UU(q), D.bc, lane);
+                wv_sweep(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 1, lane);
+                double gm = wv_max_res<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), lane);
+                if (gm < D.tol) break;
+                it += 1;
+                if (it > D.max_count) break;
+            }
+            wv_fill_ghosts(L, UU(q), D.bc, lane);
+            if (lane == 0) iters[0] = it;
         }
-        lds_fill_ghosts(L, UU(q), D.bc);
-        if (threadIdx.x == 0) iters[0] = it;
+        __syncthreads();
+        }
     }
     // up leg
     for (int q = nl - 2; q >= 0; q--) {
@@ -939,6 +1192,8 @@ static Geo geo_planes(unsigned per_plane, unsigned ip0, unsigned nplanes, unsign
 static Geo geo_sweep(const Lay &L) { return geo_planes((unsigned)(L.ny * L.kh), 1, L.nx, L.kh); }
 // both colours of the core nodes: slots (jp, c, h)
 static Geo geo_core(const Lay &L) { return geo_planes((unsigned)(L.ny * 2 * L.kh), 1, L.nx, 2 * L.kh); }
+// both colours of the core nodes, four x-planes per thread (core4_visit)
+static Geo geo_core4(const Lay &L) { return geo_planes((unsigned)(L.ny * 2 * L.kh), 0, (L.nx + 3) / 4, 2 * L.kh); }
 // every stored slot of the padded level
 static Geo geo_all(const Lay &L) { return geo_planes((unsigned)((L.ny + 2) * 2 * L.khs), 0, L.nx + 2, 2 * L.khs); }
 static const unsigned REDUCE_BLOCKS = 2048;
@@ -1012,15 +1267,6 @@ void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s) {
 int norm_parts(const Lay &L) { return (int)reduce_grid(geo_all(L)); }
 
 template <typename T>
-void launch_norm(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part,
-                 double *out, hipStream_t s) {
-    Geo G = geo_all(L);
-    int nb = (int)reduce_grid(G);
-    hipLaunchKernelGGL(k_norm_partial<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
-}
-
-template <typename T>
 void launch_rhs_sums(const Lay &L, const T *f, const double *wx, const double *wy, const double *wz, double *part,
                      double *out, hipStream_t s) {
     Geo G = geo_all(L);
@@ -1056,16 +1302,72 @@ void launch_project_small(const Lay &L, T *f, const double *wx, const double *wy
 
 void launch_outer_residual(const Lay &L, const double *u_in, const float *e, const double *f, double *u_out,
                            float *r32, const double *cx, const double *cy, const double *cz, const Bc &bc,
-                           double *part, double *out, hipStream_t s) {
-    Geo G = geo_all(L);
-    int nb = (int)reduce_grid(G);
+                           double *part2, const double *fterms, double *out, hipStream_t s) {
+    Geo G = geo_core4(L);
+    int nb = (int)G.total_blocks;
     if (e)
         hipLaunchKernelGGL(k_outer_residual<true>, dim3(nb), dim3(256), 0, s, L, G, u_in, e, f, u_out, r32, cx, cy,
-                           cz, bc, part);
+                           cz, bc, part2);
     else
         hipLaunchKernelGGL(k_outer_residual<false>, dim3(nb), dim3(256), 0, s, L, G, u_in, e, f, u_out, r32, cx, cy,
-                           cz, bc, part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, out);
+                           cz, bc, part2);
+    hipLaunchKernelGGL(k_norm_combine, dim3(1), dim3(1024), 0, s, part2, nb, fterms, out);
+}
+
+void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const double *cx,
+                     const double *cy, const double *cz, double *part5, double *coef, hipStream_t s) {
+    Geo G = geo_core4(L);
+    int nb = (int)G.total_blocks;
+    hipLaunchKernelGGL(k_gcr_dots, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);
+    hipLaunchKernelGGL(k_gcr_coef, dim3(1), dim3(1024), 0, s, part5, nb, coef);
+}
+
+void launch_gcr_update(const Lay &L, const double *x_in, const float *e, const float *z, const double *f,
+                       const double *coef, double *x_out, float *z_next, float *r32, const double *cx,
+                       const double *cy, const double *cz, const Bc &bc, const Bc &ebc, double *part2,
+                       const double *fterms, double *out, hipStream_t s) {
+    Geo G = geo_core4(L);
+    int nb = (int)G.total_blocks;
+    hipLaunchKernelGGL(k_gcr_update, dim3(nb), dim3(256), 0, s, L, G, x_in, e, z, f, coef, x_out, z_next, r32, cx,
+                       cy, cz, bc, ebc, part2);
+    hipLaunchKernelGGL(k_norm_combine, dim3(1), dim3(1024), 0, s, part2, nb, fterms, out);
+}
+
+int norm_res_parts(const Lay &L) { return (int)geo_core4(L).total_blocks; }
+
+template <typename T>
+void launch_norm_res(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part2,
+                     const double *fterms, double *out, hipStream_t s) {
+    Geo G = geo_core4(L);
+    int nb = (int)G.total_blocks;
+    hipLaunchKernelGGL(k_norm_res<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part2);
+    hipLaunchKernelGGL(k_norm_combine, dim3(1), dim3(1024), 0, s, part2, nb, fterms, out);
+}
+
+template <typename T>
+void launch_f_terms(const Lay &L, const T *f, double *part, double *fterms, hipStream_t s) {
+    Geo G = geo_all(L);
+    int nb = (int)reduce_grid(G);
+    hipLaunchKernelGGL(k_f_terms<T>, dim3(nb), dim3(256), 0, s, L, G, f, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, fterms);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_coarsest_direct(Lay L, T *__restrict__ u, const T *__restrict__ f,
+                                                          const T *__restrict__ pt, int nc, Bc bc,
+                                                          int *__restrict__ iters) {
+    long n = lay_size(L);
+    for (long t = threadIdx.x; t < n; t += blockDim.x) u[t] = T(0);
+    __syncthreads();
+    blk_direct<T>(L, u, f, pt, nc);
+    lds_fill_ghosts(L, u, bc);
+    if (threadIdx.x == 0) iters[0] = 0;
+}
+
+template <typename T>
+void launch_coarsest_direct(const Lay &L, T *u, const T *f, const T *pt, int nc, const Bc &bc, int *iters,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_coarsest_direct<T>, dim3(1), dim3(1024), 0, s, L, u, f, pt, nc, bc, iters);
 }
 
 template <typename T>
@@ -1102,8 +1404,11 @@ int coarse_cycle_attr(size_t lds) {
     template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t);               \
     template void launch_fill<T>(T *, long, T, hipStream_t);                                               \
     template void launch_add_scalar<T>(const Lay &, T *, const double *, double, int, hipStream_t);        \
-    template void launch_norm<T>(const Lay &, const T *, const T *, const T *, const T *, const T *,       \
-                                 double *, double *, hipStream_t);                                          \
+    template void launch_norm_res<T>(const Lay &, const T *, const T *, const T *, const T *, const T *,   \
+                                     double *, const double *, double *, hipStream_t);                      \
+    template void launch_f_terms<T>(const Lay &, const T *, double *, double *, hipStream_t);               \
+    template void launch_coarsest_direct<T>(const Lay &, T *, const T *, const T *, int, const Bc &, int *,    \
+                                            hipStream_t);                                                  \
     template void launch_rhs_sums<T>(const Lay &, const T *, const double *, const double *, const double *, \
                                      double *, double *, hipStream_t);                                      \
     template size_t coarsest_lds_bytes<T>(const Lay &);                                                     \

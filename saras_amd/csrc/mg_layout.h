@@ -72,6 +72,8 @@ struct CoarseDesc {
     int project;
     const double *wc[3];
     double wcsum;
+    const void *pinv;   // dense coarsest inverse, transposed (T), or null: Gauss-Seidel to tol
+    int ncd;            // coarsest core nodes (rows of pinv)
 };
 
 }  // namespace sr

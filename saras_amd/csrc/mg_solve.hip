@@ -71,6 +71,7 @@ static void load_level0(sr_mg *m, const double *drhs, const double *dp) {
         launch_rhs_sums<double>(V.L, V.f, nullptr, nullptr, nullptr, m->part, m->dres, s);
         launch_add_scalar<double>(V.L, V.f, m->dres + 1, -1.0 / npad, 0, s);
     }
+    f_terms_level0(m);
 }
 
 static int norm_order(const sr_mg *m) {
@@ -171,7 +172,7 @@ static void mixed_body_ab(sr_mg *m) {
     Level &V = m->lv[0];
     vcycle_fp32(m);
     launch_outer_residual(V.L, V.u, V.u32, V.f, V.t, V.f32, V.c64[0], V.c64[1], V.c64[2], pressure_bc(m, 0, 0),
-                          m->part, m->dres, m->stream);
+                          m->part2, m->fterms, m->dres, m->stream);
     (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
 }
 
@@ -179,7 +180,7 @@ static void mixed_body_ba(sr_mg *m) {
     Level &V = m->lv[0];
     vcycle_fp32(m);
     launch_outer_residual(V.L, V.t, V.u32, V.f, V.u, V.f32, V.c64[0], V.c64[1], V.c64[2], pressure_bc(m, 0, 0),
-                          m->part, m->dres, m->stream);
+                          m->part2, m->fterms, m->dres, m->stream);
     (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
 }
 
@@ -206,7 +207,7 @@ static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &s
     launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), m->stream);
     HIPCHK(hipMemcpyAsync(V.t, V.u, sizeof(double) * sz, hipMemcpyDeviceToDevice, m->stream));
     launch_outer_residual(V.L, V.u, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
-                          pressure_bc(m, 0, 0), m->part, m->dres, m->stream);
+                          pressure_bc(m, 0, 0), m->part2, m->fterms, m->dres, m->stream);
     HIPCHK(hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     int nt = norm_order(m);
@@ -222,6 +223,89 @@ static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &s
     for (int it = 0; it < m->d.vcCount; it++) {
         RETCHK(run_graph2(m, cur));
         HIPCHK(hipStreamSynchronize(m->stream));
+        float ms = 0.f;
+        if (m->d.preSmooth > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
+            tsum += ms;
+            tcount += 2 * m->d.preSmooth;
+        }
+        cur ^= 1;
+        dsol = cur ? V.t : V.u;
+        r = error_from(m, m->hres, nt);
+        m->history.push_back(r);
+        cycles = it + 1;
+        res = r;
+        if (m->d.semantics == SR_SEM_PROJECTED) {
+            if (r < m->d.relTolerance) { conv = true; break; }
+        } else if (!m->d.testPoisson) {
+            if (std::fabs(prev - r) < m->d.mgTolerance) { conv = true; break; }
+            prev = r;
+        }
+    }
+    if (m->d.testPoisson && m->d.semantics != SR_SEM_PROJECTED) conv = true;
+    sweep_ms = tcount ? tsum / tcount : 0.0;
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Krylov-accelerated mixed precision: flexible GCR(1) with one FP32 V-cycle as the preconditioner.
+// The iterate ping-pongs between the level-0 u and tmp buffers like the refinement loop, the FP32
+// search direction between zdir[0] and zdir[1]; the step coefficients stay on the device, so one
+// iteration is one captured graph with a single read-back of the residual norms.
+// ------------------------------------------------------------------------------------------------
+static void gcr_body(sr_mg *m, int cur) {
+    Level &V = m->lv[0];
+    hipStream_t s = m->stream;
+    double *x_in = cur ? V.t : V.u, *x_out = cur ? V.u : V.t;
+    const float *z = m->zdir[cur];
+    float *z_next = m->zdir[cur ^ 1];
+    vcycle_fp32(m);
+    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c64[0], V.c64[1], V.c64[2], m->gpart, m->gcoef, s);
+    launch_gcr_update(V.L, x_in, V.u32, z, V.f, m->gcoef, x_out, z_next, V.f32, V.c64[0], V.c64[1], V.c64[2],
+                      pressure_bc(m, 0, 0), error_bc(m, 0, 0), m->part2, m->fterms, m->dres, s);
+    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
+}
+static void gcr_body_0(sr_mg *m) { gcr_body(m, 0); }
+static void gcr_body_1(sr_mg *m) { gcr_body(m, 1); }
+
+static int run_gcr_graph(sr_mg *m, int which) {
+    void (*body)(sr_mg *) = which == 0 ? gcr_body_0 : gcr_body_1;
+    hipGraphExec_t &ex = m->graph_gcr[which];
+    if (!ex && !m->graphs_off) capture(m, body, m->graph_gcr_g[which], ex);
+    if (!ex) {
+        body(m);
+        HIPCHK(hipGetLastError());
+        return SR_OK;
+    }
+    HIPCHK(hipGraphLaunch(ex, m->stream));
+    return SR_OK;
+}
+
+static int solve_gcr(sr_mg *m, int &cycles, double &res, bool &conv, double &sweep_ms, double *&dsol) {
+    if (!m->d.gsSmooth) return set_err(SR_ERR_ARG, "mixed precision needs gsSmooth = 1");
+    Level &V = m->lv[0];
+    long sz = lay_size(V.L);
+    hipStream_t s = m->stream;
+    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), s);
+    HIPCHK(hipMemcpyAsync(V.t, V.u, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+    // no previous direction: the first step is the minimal-residual step along the cycle's result
+    launch_fill<float>(m->zdir[0], sz, 0.0f, s);
+    launch_outer_residual(V.L, V.u, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
+                          pressure_bc(m, 0, 0), m->part2, m->fterms, m->dres, s);
+    HIPCHK(hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int nt = norm_order(m);
+    double r = error_from(m, m->hres, nt);
+    dsol = V.u;
+    conv = false;
+    cycles = 0;
+    res = r;
+    if (m->d.semantics == SR_SEM_PROJECTED && r < m->d.relTolerance) { conv = true; return SR_OK; }
+    double prev = 0.0, tsum = 0.0;
+    int tcount = 0;
+    int cur = 0;
+    for (int it = 0; it < m->d.vcCount; it++) {
+        RETCHK(run_gcr_graph(m, cur));
+        HIPCHK(hipStreamSynchronize(s));
         float ms = 0.f;
         if (m->d.preSmooth > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
             tsum += ms;
@@ -293,6 +377,8 @@ extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device
         RETCHK(solve_fp64(m, cycles, res, conv, sweep_ms));
     } else if (m->d.precision == SR_PREC_MIXED) {
         RETCHK(solve_mixed(m, cycles, res, conv, sweep_ms, dsol));
+    } else if (m->d.precision == SR_PREC_MIXED_GCR) {
+        RETCHK(solve_gcr(m, cycles, res, conv, sweep_ms, dsol));
     } else {
         return set_err(SR_ERR_ARG, "unknown precision mode");
     }
@@ -366,7 +452,9 @@ extern "C" int sr_mg_time_kernel(sr_mg *m, int kernel, int level, int reps, doub
             case 2: launch_restrict<double>(V.L, m->lv[level + 1].L, V.t, m->lv[level + 1].f, s); break;
             case 3: launch_prolong<double, double>(V.L, m->lv[level + 1].L, m->lv[level + 1].u, V.u, 1, s); break;
             case 4: launch_fill_ghosts<double>(V.L, V.u, bc, s); break;
-            case 5: launch_norm<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part, m->dres, s); break;
+            case 5:
+                launch_norm_res<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part2, m->fterms, m->dres, s);
+                break;
             case 6:
                 launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], r & 1, bc, false, s);
                 break;

@@ -43,6 +43,8 @@ struct sr_mg {
     double *wvec[3] = {nullptr, nullptr, nullptr};
     double wsum = 0;
     double *part = nullptr;      // reduction partials
+    double *part2 = nullptr;     // per-block (sum, max) partials of the level-0 residual norm
+    double *fterms = nullptr;    // right-hand side terms of computeError (device, 4 doubles)
     double *dres = nullptr;      // reduced quantities (device)
     double *hres = nullptr;      // pinned host mirror
     int *diters = nullptr;
@@ -59,6 +61,17 @@ struct sr_mg {
     long lds_limit = 0;
     hipGraph_t graph_mixed_g[2] = {nullptr, nullptr};
     hipGraphExec_t graph_mixed[2] = {nullptr, nullptr};
+    // Krylov-accelerated mixed precision: FP32 search directions (ping-pong), block sums, step
+    // coefficients and the two captured iteration bodies
+    // exact coarsest solve (coarseDirect): dense inverse, transposed, FP64 and FP32 copies
+    double *pinv64 = nullptr;
+    float *pinv32 = nullptr;
+    int ncd = 0;
+    float *zdir[2] = {nullptr, nullptr};
+    double *gpart = nullptr;
+    double *gcoef = nullptr;
+    hipGraph_t graph_gcr_g[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_gcr[2] = {nullptr, nullptr};
     std::vector<double> history;
 };
 
@@ -68,6 +81,8 @@ int set_err(int code, const std::string &msg);
 void mark(sr_mg *m, hipEvent_t e);
 Bc pressure_bc(const sr_mg *m, int level, int zeroBC);
 Bc tmp_bc(const sr_mg *m);
+// ghost rules of a correction (pressure rules with homogeneous walls)
+Bc error_bc(const sr_mg *m, int l, int zeroBC);
 void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
                   hipEvent_t ea = nullptr, hipEvent_t eb = nullptr);
 void residual_level(sr_mg *m, int l);
@@ -75,7 +90,8 @@ void coarsest_level(sr_mg *m, int l, int zeroBC);
 void vcycle_fp64(sr_mg *m);
 void vcycle_fp32(sr_mg *m);
 int ensure_fp32(sr_mg *m);
-void norm_level0(sr_mg *m);
+void norm_level0(sr_mg *m);       // needs the right-hand side terms of f_terms_level0
+void f_terms_level0(sr_mg *m);
 double error_from(const sr_mg *m, const double *q, int norm);
 
 template <typename T>

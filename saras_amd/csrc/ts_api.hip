@@ -54,8 +54,12 @@ struct sr_ts {
     // scalar_d3
     double *T = nullptr, *Trhs = nullptr, *Tg = nullptr;
     std::vector<GhostWave> tbc;
-    GhostWave sync_t{};
     double nu = 0.0, kappa = 0.0, Fb = 0.0;
+    // learned initial guess of the pressure solve
+    sr_guess *guess = nullptr;
+    int train = 0;
+    double *Pp1 = nullptr, *Pp2 = nullptr;   // previous two corrections
+    long nsolved = 0;
 };
 
 static long fsize(const FShape &f) { return (long)(f.n[0] + 2) * (f.n[1] + 2) * (f.n[2] + 2); }
@@ -288,7 +292,8 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
     md.gsSmooth = d.gsSmooth; md.resType = d.resType;
     md.xPer = d.xPer; md.yPer = d.yPer; md.zPer = d.zPer;
     md.semantics = d.semantics;
-    md.precision = SR_PREC_FP64;
+    md.precision = d.precision;
+    md.coarseDirect = d.coarseDirect;
     md.mgTolerance = d.mgTolerance;
     md.relTolerance = d.relTolerance > 0.0 ? d.relTolerance : 1e-10;
     md.dXi = t->h[0]; md.dEt = t->h[1]; md.dZt = t->h[2];
@@ -328,7 +333,8 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
         ts_fill(t->Tg, np, 0.0, t->s);
         t->tbc = tbc_waves(d, t->fp);
     }
-    HIPCHK(hipMalloc((void **)&t->part, sizeof(double) * 2 * (size_t)ts_energy_parts(t->fp)));
+    HIPCHK(hipMalloc((void **)&t->part,
+                     sizeof(double) * std::max<size_t>(2 * (size_t)ts_energy_parts(t->fp), ts_jacobi_res_parts())));
     HIPCHK(hipMalloc((void **)&t->dred, sizeof(double) * 4));
     HIPCHK(hipHostMalloc((void **)&t->hred, sizeof(double) * 4, hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&t->dmax, sizeof(unsigned long long)));
@@ -346,7 +352,8 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
 extern "C" int sr_ts_destroy(sr_ts *t) {
     if (!t) return SR_OK;
     if (t->s) (void)hipStreamSynchronize(t->s);
-    double *bufs[] = {t->dmet, t->P, t->Pp, t->mgRHS, t->divV, t->scratch, t->part, t->dred, t->T, t->Trhs, t->Tg};
+    double *bufs[] = {t->dmet, t->P,    t->Pp,   t->mgRHS, t->divV, t->scratch, t->part,
+                      t->dred, t->T,    t->Trhs, t->Tg,    t->Pp1,  t->Pp2};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     for (int c = 0; c < 3; c++) {
@@ -389,6 +396,12 @@ static int field(sr_ts *t, int which, double **ptr, const FShape **shape) {
         case SR_TS_HX: case SR_TS_HY: case SR_TS_HZ:
             *ptr = t->H[which - SR_TS_HX]; *shape = &t->fv[which - SR_TS_HX]; return SR_OK;
         case SR_TS_DIV: *ptr = t->divV; *shape = &t->fp; return SR_OK;
+        case SR_TS_PP1:
+        case SR_TS_PP2:
+            if (!t->Pp1) return set_err(SR_ERR_STATE, "no correction history (no guess attached)");
+            *ptr = which == SR_TS_PP1 ? t->Pp1 : t->Pp2;
+            *shape = &t->fp;
+            return SR_OK;
         case SR_TS_T:
         case SR_TS_TRHS:
             if (!t->T) return set_err(SR_ERR_STATE, "no temperature field (desc.scalar = 0)");
@@ -520,8 +533,7 @@ static int jacobi(sr_ts *t, const FShape &f, double *&F, double *&G, const doubl
         ts_jacobi(f, t->M, B, F, R, G, J, t->s);
         std::swap(F, G);
         for (const GhostWave &w : bc) ts_ghost_wave(f, F, w, t->s);
-        ts_clear_u64(t->dmax, t->s);
-        ts_jacobi_res(f, t->M, B, F, R, J, t->dmax, t->s);
+        ts_jacobi_res(f, t->M, B, F, R, J, t->part, t->dmax, t->s);
         HIPCHK(hipMemcpyAsync(t->hmax, t->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, t->s));
         HIPCHK(hipStreamSynchronize(t->s));
         HIPCHK(hipGetLastError());
@@ -618,10 +630,28 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     // mgRHS = div V / dt (hydro_d3.cc:336-340); pads stay 0
     ts_divergence(t->fp, t->fv, t->M, Vc, t->mgRHS, t->h, 1.0 / dt, 1, t->s);
     HIPCHK(hipGetLastError());
+    double guess_ms = 0.0, rel_guess = -1.0, loss = -1.0;
+    int choice = -1;
+    if (t->guess && t->nsolved >= 1) {
+        HIPCHK(hipStreamSynchronize(t->s));
+        sr_guess_stats gs;
+        RETCHK(sr_guess_predict(t->guess, t->mg, t->Pp1, t->nsolved >= 2 ? t->Pp2 : nullptr, t->mgRHS, t->Pp, &gs));
+        choice = gs.choice;
+        guess_ms = gs.predict_ms;
+        rel_guess = choice == 2 ? gs.rel_learned : choice == 1 ? gs.rel_extrap : gs.rel_warm;
+    }
     sr_mg_stats ms;
+    auto ts0 = std::chrono::steady_clock::now();
     RETCHK(sr_mg_solve(t->mg, t->Pp, t->mgRHS, 1, &ms));
+    auto ts1 = std::chrono::steady_clock::now();
     t->mg_cycles = ms.cycles;
+    if (t->guess && choice >= 0 && t->train) RETCHK(sr_guess_train(t->guess, t->Pp, &loss));
     ts_ghost_wave(t->fp, t->Pp, t->sync_p, t->s);
+    if (t->guess) {
+        std::swap(t->Pp1, t->Pp2);
+        HIPCHK(hipMemcpyAsync(t->Pp1, t->Pp, sizeof(double) * fsize(t->fp), hipMemcpyDeviceToDevice, t->s));
+    }
+    t->nsolved += 1;
     ts_axpy_full(t->P, t->Pp, fsize(t->fp), t->s);
     for (int c = 0; c < 3; c++) ts_project(c, t->fv[c], t->fp, t->M, t->Pp, t->V[c], t->h[c], dt, t->s);
     if (t->d.scalar) {
@@ -638,6 +668,11 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
         for (int c = 0; c < 4; c++) st->jacobi_iters[c] = t->jac[c];
+        st->solve_ms = std::chrono::duration<double, std::milli>(ts1 - ts0).count();
+        st->guess_ms = guess_ms;
+        st->guess_choice = choice;
+        st->rel_guess = rel_guess;
+        st->train_loss = loss;
         st->mg_cycles = t->mg_cycles;
         st->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
@@ -712,5 +747,22 @@ extern "C" int sr_ts_nusselt(sr_ts *t, double *nu) {
     HIPCHK(hipStreamSynchronize(t->s));
     HIPCHK(hipGetLastError());
     *nu = 1.0 + (t->hred[1] / t->hred[0]) / t->kappa;
+    return SR_OK;
+}
+
+extern "C" int sr_ts_attach_guess(sr_ts *t, sr_guess *g, int train) {
+    if (!t) return set_err(SR_ERR_ARG, "null argument");
+    t->guess = g;
+    t->train = train;
+    if (g && !t->Pp1) {
+        const long np = fsize(t->fp);
+        HIPCHK(hipMalloc((void **)&t->Pp1, sizeof(double) * np));
+        HIPCHK(hipMalloc((void **)&t->Pp2, sizeof(double) * np));
+        ts_fill(t->Pp1, np, 0.0, t->s);
+        ts_fill(t->Pp2, np, 0.0, t->s);
+        t->nsolved = 0;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(t->s));
+    }
     return SR_OK;
 }

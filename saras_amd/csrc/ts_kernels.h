@@ -66,9 +66,11 @@ void ts_nusselt(const FShape &fp, const FShape &fz, const Metrics &M, const doub
                 const double *h, double *part, double *out, hipStream_t s);
 void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
                const JacobiCoef &J, hipStream_t s);
+// max |residual| of the Jacobi solve over the bulk, as the bit pattern of the double in out[0];
+// part holds ts_jacobi_res_parts() doubles
+int ts_jacobi_res_parts();
 void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R,
-                   const JacobiCoef &J, unsigned long long *out, hipStream_t s);
-void ts_clear_u64(unsigned long long *p, hipStream_t s);
+                   const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s);
 void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s);
 void ts_divergence(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, double *out,
                    const double *h, double scale, int scaled, hipStream_t s);

@@ -155,13 +155,15 @@ __global__ void __launch_bounds__(256) k_jacobi_v(FShape f, Metrics M, Bulk B, c
                                                   JacobiCoef J) {
     // one thread per padded entry: the bulk gets the Jacobi update, every other entry 0 (the
     // reference's guess field is written on the bulk only and copied whole into V)
-    long total = (long)(f.n[0] + 2) * (f.n[1] + 2) * (f.n[2] + 2);
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned n1 = (unsigned)(f.n[1] + 2), n2 = (unsigned)(f.n[2] + 2);
+    unsigned total = (unsigned)(f.n[0] + 2) * n1 * n2;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    int k = (int)(t % (f.n[2] + 2)) - 1;
-    long r = t / (f.n[2] + 2);
-    int j = (int)(r % (f.n[1] + 2)) - 1;
-    int i = (int)(r / (f.n[1] + 2)) - 1;
+    unsigned r = t / n2;
+    int k = (int)(t - r * n2) - 1;
+    unsigned i0 = r / n1;
+    int j = (int)(r - i0 * n1) - 1;
+    int i = (int)i0 - 1;
     if (i < B.lo[0] || i > B.hi[0] || j < B.lo[1] || j > B.hi[1] || k < B.lo[2] || k > B.hi[2]) {
         G[t] = 0.0;
         return;
@@ -178,38 +180,48 @@ __global__ void __launch_bounds__(256) k_jacobi_v(FShape f, Metrics M, Bulk B, c
         G[q] = (a * J.dt * J.vis / (J.hx2hy2hz2 * 2.0) + R[q]) / (1.0 + J.dt * J.vis * (d) / J.hx2hy2hz2);
 }
 
-// max over the bulk of |V - 0.5 dt/Re lap V - R|; maxima of non-negative doubles ordered as their
-// bit patterns, so the atomic maximum is exact and independent of order
+// max over the bulk of |V - 0.5 dt/Re lap V - R|: grid-stride over the bulk (32-bit index
+// arithmetic), one maximum per block into part; k_jacobi_res_final reduces them
 __global__ void __launch_bounds__(256) k_jacobi_res(FShape f, Metrics M, Bulk B, const double *__restrict__ V,
                                                     const double *__restrict__ R, JacobiCoef J,
-                                                    unsigned long long *__restrict__ out) {
-    int sz[3] = {B.hi[0] - B.lo[0] + 1, B.hi[1] - B.lo[1] + 1, B.hi[2] - B.lo[2] + 1};
-    long total = (long)sz[0] * sz[1] * sz[2];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                    double *__restrict__ part) {
+    unsigned sz1 = (unsigned)(B.hi[1] - B.lo[1] + 1), sz2 = (unsigned)(B.hi[2] - B.lo[2] + 1);
+    unsigned total = (unsigned)(B.hi[0] - B.lo[0] + 1) * sz1 * sz2;
+    long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
     double e = 0.0;
-    if (t < total) {
-        int k = (int)(t % sz[2]) + B.lo[2];
-        long r = t / sz[2];
-        int j = (int)(r % sz[1]) + B.lo[1];
-        int i = (int)(r / sz[1]) + B.lo[0];
-        long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        unsigned r = t / sz2;
+        int k = (int)(t - r * sz2) + B.lo[2];
+        unsigned i0 = r / sz1;
+        int j = (int)(r - i0 * sz1) + B.lo[1];
+        int i = (int)i0 + B.lo[0];
         long q = fidx(f, i, j, k);
         double mx = metv(M, 0, J.ms[0], 3, i), my = metv(M, 1, J.ms[1], 3, j), mz = metv(M, 2, J.ms[2], 3, k);
         double lap = V[q] - (mx * (V[q + st[0]] - 2.0 * V[q] + V[q - st[0]]) / (J.hx * J.hx) +
                              my * (V[q + st[1]] - 2.0 * V[q] + V[q - st[1]]) / (J.hy * J.hy) +
                              mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (J.hz * J.hz)) * 0.5 * J.dt *
                                 (J.form == 0 ? J.inverseRe : J.vis);
-        e = fabs(lap - R[q]);
+        e = fmax(e, fabs(lap - R[q]));
     }
     __shared__ double s[4];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = e;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        e = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
-        atomicMax(out, (unsigned long long)__double_as_longlong(e));
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+}
+
+// maximum of the block maxima, stored as the bit pattern of the double
+__global__ void __launch_bounds__(256) k_jacobi_res_final(const double *__restrict__ part, int n,
+                                                          unsigned long long *__restrict__ out) {
+    double m = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
+    __shared__ double s[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = (unsigned long long)__double_as_longlong(fmax(fmax(s[0], s[1]), fmax(s[2], s[3])));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -370,8 +382,6 @@ __global__ void __launch_bounds__(256) k_project(int c, FShape fc, FShape fp, Me
     V[q] = V[q] - g;
 }
 
-__global__ void k_clear_u64(unsigned long long *p) { p[threadIdx.x] = 0ull; }
-
 __global__ void __launch_bounds__(256) k_fill_d(double *__restrict__ p, long n, double v) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -531,10 +541,16 @@ void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V
     hipLaunchKernelGGL(k_jacobi_v, dim3(nb(n)), dim3(256), 0, s, f, M, B, V, R, G, J);
 }
 
+int ts_jacobi_res_parts() { return 1024; }
+
 void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R,
-                   const JacobiCoef &J, unsigned long long *out, hipStream_t s) {
+                   const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s) {
     long n = (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
-    hipLaunchKernelGGL(k_jacobi_res, dim3(nb(n)), dim3(256), 0, s, f, M, B, V, R, J, out);
+    unsigned g = nb(n);
+    if (g > (unsigned)ts_jacobi_res_parts()) g = (unsigned)ts_jacobi_res_parts();
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_jacobi_res, dim3(g), dim3(256), 0, s, f, M, B, V, R, J, part);
+    hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(256), 0, s, part, (int)g, out);
 }
 
 void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s) {
@@ -560,7 +576,6 @@ void ts_axpy_full(double *y, const double *x, long n, hipStream_t s) {
     hipLaunchKernelGGL(k_axpy_full, dim3(nb(n)), dim3(256), 0, s, y, x, n);
 }
 
-void ts_clear_u64(unsigned long long *p, hipStream_t s) { hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(1), 0, s, p); }
 
 void ts_fill(double *p, long n, double v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_d, dim3(nb(n)), dim3(256), 0, s, p, n, v);

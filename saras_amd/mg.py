@@ -47,6 +47,7 @@ class Parser:
     precision: int = _L.SR_PREC_FP64
     padMode: int = 0
     coarseProject: int = 0
+    coarseDirect: int = 0
     relTolerance: float = 1.0e-10
 
     @staticmethod
@@ -110,7 +111,7 @@ class Multigrid:
         d.gsSmooth, d.resType = int(p.gsSmooth), p.resType
         d.xPer, d.yPer, d.zPer = int(p.xPer), int(p.yPer), int(p.zPer)
         d.testPoisson, d.semantics, d.precision = int(p.testPoisson), p.semantics, p.precision
-        d.padMode, d.coarseProject = p.padMode, p.coarseProject
+        d.padMode, d.coarseProject, d.coarseDirect = p.padMode, p.coarseProject, p.coarseDirect
         d.mgTolerance, d.relTolerance = p.mgTolerance, p.relTolerance
         d.dXi, d.dEt, d.dZt = grid.dXi, grid.dEt, grid.dZt
         for name, arr in (("xix2", grid.xix2Staggr), ("xixx", grid.xixxStaggr), ("ety2", grid.ety2Staggr),

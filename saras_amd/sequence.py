@@ -11,6 +11,10 @@ below differ only in the initial guess handed to the multigrid solve and in its 
   learned_mixed mixed precision from the learned guess (include/saras_guess.h), trained online on
                 the solves of the first `ntrain` steps, then frozen
   learned_fp64  FP64 multigrid from the learned guess
+  gcr_warm      FP32 V-cycle preconditioned GCR(1) under the FP64 residual (SR_PREC_MIXED_GCR),
+                from the previous solution
+  gcr_extrap    the same from 2 p_{n-1} - p_{n-2}
+  learned_gcr   the same from the learned guess
 
 Each solve runs to the same stop rule (relative residual 1e-10 on the projected right-hand side);
 the time of a step is the guess plus the solve, with the right-hand side already in HBM."""
@@ -22,7 +26,16 @@ import numpy as np
 from . import _lib as _L
 from . import mg
 
-VARIANTS = ("fp64_zero", "fp64_warm", "mixed_warm", "mixed_extrap", "learned_mixed", "learned_fp64")
+VARIANTS = ("fp64_zero", "fp64_warm", "mixed_warm", "mixed_extrap", "learned_mixed", "learned_fp64", "gcr_warm",
+            "gcr_extrap", "learned_gcr")
+
+
+def precision_of(variant):
+    if "gcr" in variant:
+        return _L.SR_PREC_MIXED_GCR
+    if "mixed" in variant:
+        return _L.SR_PREC_MIXED
+    return _L.SR_PREC_FP64
 
 
 def net_grid_for(shape):
@@ -35,7 +48,7 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None)
     per-step records and the network handle used (if any)."""
     assert variant in VARIANTS, variant
     p = copy.copy(seq.p)
-    p.precision = _L.SR_PREC_MIXED if variant.startswith("mixed") or variant == "learned_mixed" else _L.SR_PREC_FP64
+    p.precision = precision_of(variant)
     solver = mg.Multigrid(seq.g, p)
     shape = seq.shape
     d_f = _L.DeviceArray(shape)
@@ -43,7 +56,7 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None)
     d_zero.upload(np.zeros(shape))
     ring = [_L.DeviceArray(shape) for _ in range(3)]
     lib = _L.load()
-    need_net = variant in ("mixed_extrap", "learned_mixed", "learned_fp64")
+    need_net = variant.endswith("extrap") or variant.startswith("learned")
     if need_net and net is None:
         from .guess import GuessNet
         net = GuessNet(net_grid_for(shape), seed=seed, lr=lr, fine=tuple(k - 2 for k in shape))
@@ -59,7 +72,7 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None)
             _L.check(lib.sr_memcpy(cur.ptr, d_zero.ptr, cur.nbytes, 3))
         elif variant.endswith("warm"):
             _L.check(lib.sr_memcpy(cur.ptr, p1.ptr, cur.nbytes, 3))
-        elif variant == "mixed_extrap":
+        elif variant.endswith("extrap"):
             net.extrapolate(p1.ptr, p2.ptr if p2 else None, cur.ptr)
         else:
             st = net.predict(solver, p1.ptr, p2.ptr if p2 else None, d_f.ptr, cur.ptr)

@@ -12,6 +12,7 @@ from . import _lib as _L
 VX, VY, VZ, P, PP, MGRHS = _L.SR_TS_VX, _L.SR_TS_VY, _L.SR_TS_VZ, _L.SR_TS_P, _L.SR_TS_PP, _L.SR_TS_MGRHS
 HX, HY, HZ, DIV = _L.SR_TS_HX, _L.SR_TS_HY, _L.SR_TS_HZ, _L.SR_TS_DIV
 T, TRHS = _L.SR_TS_T, _L.SR_TS_TRHS
+PP1, PP2 = _L.SR_TS_PP1, _L.SR_TS_PP2
 
 
 def _ptr(a):
@@ -20,12 +21,14 @@ def _ptr(a):
 
 def ts_desc(ind=(7, 7, 7), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1.0, 1.0, 1.0), Re=1600.0,
             dt=1e-3, cn_tol=1e-6, depth=None, vc_count=5, pre=4, post=4, res_type=1, mg_tol=1e-6, prob_type=2,
-            scalar=False, rbc_type=2, Ra=1e7, Pr=1.0, semantics=_L.SR_SEM_REFERENCE, rel_tol=1e-10):
+            scalar=False, rbc_type=2, Ra=1e7, Pr=1.0, semantics=_L.SR_SEM_REFERENCE, rel_tol=1e-10,
+            precision=_L.SR_PREC_FP64, coarse_direct=0):
     """The parser fields of a hydro (or, with scalar=True, scalar) run (input/parameters.yaml keys;
     defaults: the Taylor-Green case)."""
     d = _L.SrTsDesc()
     d.scalar, d.rbcType, d.Ra, d.Pr = int(scalar), rbc_type, Ra, Pr
-    d.semantics, d.relTolerance = semantics, rel_tol
+    d.semantics, d.relTolerance, d.precision = semantics, rel_tol, precision
+    d.coarseDirect = int(coarse_direct)
     d.xInd, d.yInd, d.zInd = ind
     d.Lx, d.Ly, d.Lz = L
     d.xGrid, d.yGrid, d.zGrid = (1 if c == "U" else 2 for c in mesh)
@@ -144,3 +147,9 @@ class Hydro:
         v = C.c_double()
         _L.check(self.L.sr_ts_nusselt(self.h, C.byref(v)))
         return v.value
+
+    # -- learned initial guess of the pressure solve -------------------------------------------
+    def attach_guess(self, net, train=True):
+        """net: saras_amd.guess.GuessNet whose solver grid is this run's node grid (or None)."""
+        _L.check(self.L.sr_ts_attach_guess(self.h, net.h if net is not None else None, int(train)))
+        self._guess = net

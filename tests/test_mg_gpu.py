@@ -266,22 +266,28 @@ def test_257_cubed_projected_solve_reaches_tolerance():
 # ------------------------------------------------------------------------------------------------
 # mixed precision: FP32 inner V-cycles under FP64 iterative refinement
 # ------------------------------------------------------------------------------------------------
-def _mixed_pair(ind, mesh="UUD", domain="PPN", **kw):
+def _mixed_pair(ind, mesh="UUD", domain="PPN", precision=None, **kw):
     from saras_amd import _lib
     pb = Problem(ind=ind, domain=domain, mesh=mesh, semantics=po.SEM_PROJECTED, rel_tol=1e-10, **kw)
-    return pb, pb.gpu(), pb.gpu(precision=_lib.SR_PREC_MIXED)
+    return pb, pb.gpu(), pb.gpu(precision=_lib.SR_PREC_MIXED if precision is None else precision)
 
 
-@pytest.mark.parametrize("ind", [(5, 5, 5), (6, 6, 5)])
-def test_mixed_precision_reaches_fp64_solution(ind):
-    pb, g64, gm = _mixed_pair(ind)
+@pytest.mark.parametrize("gcr", [False, True], ids=["refinement", "gcr"])
+@pytest.mark.parametrize("ind,domain", [((5, 5, 5), "PPN"), ((6, 6, 5), "PPN"), ((5, 5, 5), "NNN")])
+def test_mixed_precision_reaches_fp64_solution(ind, domain, gcr):
+    from saras_amd import _lib
+    pb, g64, gm = _mixed_pair(ind, domain=domain, precision=_lib.SR_PREC_MIXED_GCR if gcr else None)
     f = pb.random_rhs(21)
     p64 = np.zeros(pb.shape)
     s64 = g64.mgSolve(p64, f)
     pm = np.zeros(pb.shape)
     sm = gm.mgSolve(pm, f)
     assert s64.converged and sm.converged and sm.residual < 1e-10
-    assert abs(sm.cycles - s64.cycles) <= 2
+    if gcr:
+        # the Krylov step never costs cycles against plain refinement
+        assert sm.cycles <= s64.cycles
+    else:
+        assert abs(sm.cycles - s64.cycles) <= 2
     # both solve the same singular system: compare up to the constant null vector
     a, b = pm[1:-1, 1:-1, 1:-1], p64[1:-1, 1:-1, 1:-1]
     d = (a - a.mean()) - (b - b.mean())
@@ -294,11 +300,12 @@ def test_mixed_precision_reaches_fp64_solution(ind):
     assert o.compute_error(2) < 1.2e-10
 
 
-def test_mixed_precision_dirichlet_r2():
+@pytest.mark.parametrize("gcr", [False, True], ids=["refinement", "gcr"])
+def test_mixed_precision_dirichlet_r2(gcr):
     from saras_amd import _lib, mg
     n = (33, 33, 33)
     pr = mg.Parser(xInd=5, yInd=5, zInd=5, vcDepth=4, vcCount=40, testPoisson=True, semantics=_lib.SR_SEM_PROJECTED,
-                   precision=_lib.SR_PREC_MIXED, relTolerance=1e-10)
+                   precision=_lib.SR_PREC_MIXED_GCR if gcr else _lib.SR_PREC_MIXED, relTolerance=1e-10)
     gm = mg.Multigrid(mg.Grid(pr), pr, walls=r2_walls(n))
     f = np.zeros(tuple(k + 2 for k in n))
     f[1:-1, 1:-1, 1:-1] = 1.0
@@ -306,6 +313,25 @@ def test_mixed_precision_dirichlet_r2():
     st = gm.mgSolve(p, f)
     assert st.converged
     assert np.max(np.abs(p[1:-1, 1:-1, 1:-1] - r2_exact(n))) < 1e-8
+
+
+def test_gcr_cuts_cycles_at_257_cubed():
+    """Config C2 at full size: the Krylov-accelerated mixed solve reaches 1e-10 in at most three
+    quarters of the plain FP64 cycles, and the oracle's computeError confirms its FP64 residual."""
+    from saras_amd import _lib
+    pb = Problem(ind=(8, 8, 8), domain="PPN", mesh="UUD", depth=7, semantics=po.SEM_PROJECTED, rel_tol=1e-10,
+                 vc_count=80)
+    f = pb.random_rhs(14)
+    p64, pg = np.zeros(pb.shape), np.zeros(pb.shape)
+    s64 = pb.gpu().mgSolve(p64, f)
+    sg = pb.gpu(precision=_lib.SR_PREC_MIXED_GCR).mgSolve(pg, f)
+    assert s64.converged and sg.converged and sg.residual < 1e-10
+    assert sg.cycles <= 0.75 * s64.cycles, (sg.cycles, s64.cycles)
+    o = pb.oracle()
+    o.level = 0
+    o.array(po.P, 0)[...] = pg
+    o.array(po.R, 0)[...] = o.project_rhs(f.copy())
+    assert o.compute_error(2) < 1.2e-10
 
 
 def test_coarse_cycle_kernel_is_planned_and_matches_per_level_path():
