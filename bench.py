@@ -1,11 +1,18 @@
 """Benchmark: ms per Poisson solve to 1e-10 relative residual on the 257^3 stretched grid.
 
-Workload (one "step" = one solve): 257^3 nodes, periodic x/y, Neumann z, tanh z grid (beta 1),
+Grid of every Poisson workload: 257^3 nodes, periodic x/y, Neumann z, tanh z grid (beta 1),
 right-hand side = 8 seeded Fourier modes (projected onto the operator's range by the solver),
-V(2,2) red-black Gauss-Seidel cycles with vcDepth 7 from a zero initial guess until
-||f - L u||_2 / ||f||_2 < 1e-10.  Inputs are resident in HBM before the timed region.
+V(2,2) red-black Gauss-Seidel cycles with vcDepth 7, stop on ||f - L u||_2 / ||f||_2 < 1e-10.
 
-Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one GPU.
+  seq257      (default) one step = one solve of a sequence whose right-hand side evolves in time;
+              learned guess + mixed precision against plain FP64 multigrid (seq_bench)
+  poisson257  one step = one solve of the single right-hand side from a zero guess: plain FP64,
+              mixed-precision refinement, GCR-accelerated mixed precision
+  tgv129      one step = one Taylor-Green hydro_d3 time step at 129^3
+  rbc257x129  one step = one Rayleigh-Benard scalar_d3 time step (pressure solve timed)
+
+Inputs are resident in HBM before the timed region.  Prints one JSON line.  ``--gpus`` above 1 is
+refused: this build runs on one GPU.
 """
 import argparse
 import json
@@ -18,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/Poisson solve to 1e-10 rel. residual, 257³ (learned+mixed vs FP64 MG)"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (KiB per dispatch) scaled by what the same counters read for a device copy of known size
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the summaries and commands are in profiles/
+TRAFFIC_BYTES = {"k_rb_sweep<float> 257^3": 123.9e6, "k_rb_sweep<double> 257^3": 228.1e6}
 
 
 def cpu_baseline(p, g, f, max_cycles):
@@ -125,7 +136,7 @@ def rbc_bench(args):
               res_type=2, depth=nz_ind - 1)
     n = (257, 257, 257 if nz_ind == 8 else 129)
     net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3)
-    run = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, **kw))
+    run = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1, **kw))
     run.attach_guess(net, train=True)
     run.initRBC(seed=0, amp=1e-3)
     t0 = time.perf_counter()
@@ -149,9 +160,9 @@ def rbc_bench(args):
     nu_end = run.nusselt()
     run.close()
     out = {}
-    for name, prec in (("fp64_warm", _lib.SR_PREC_FP64), ("mixed_warm", _lib.SR_PREC_MIXED),
-                       ("gcr_warm", _lib.SR_PREC_MIXED_GCR)):
-        h = ts.Hydro(ts.ts_desc(precision=prec, **kw))
+    for name, prec, direct in (("fp64_warm", _lib.SR_PREC_FP64, 0), ("mixed_warm", _lib.SR_PREC_MIXED, 0),
+                               ("gcr_warm", _lib.SR_PREC_MIXED_GCR, 1)):
+        h = ts.Hydro(ts.ts_desc(precision=prec, coarse_direct=direct, **kw))
         h.initRBC(seed=0, amp=1e-3)
         for w, a in snap.items():
             h.upload(w, a)
@@ -173,7 +184,8 @@ def rbc_bench(args):
                                "1e-10 rel. residual; guess trained online over the warm-up steps",
                    "learned_mixed": learned, "fp64_warm": out["fp64_warm"], "mixed_warm": out["mixed_warm"],
                    "gcr_warm": out["gcr_warm"],
-                   "mixed_solver": "FP32 V(2,2) cycle preconditioning flexible GCR(1) under the FP64 residual",
+                   "mixed_solver": "FP32 V(2,2) cycle preconditioning flexible GCR(1) under the FP64 residual, "
+                                   "exact coarsest level",
                    "speedup_learned_mixed_vs_fp64_warm": round(base / learned["ms_per_solve"], 3),
                    "train_seconds": round(train_s, 2), "train_loss_first": train_loss[:3],
                    "train_loss_last": train_loss[-3:], "nusselt_end": nu_end},
@@ -182,20 +194,116 @@ def rbc_bench(args):
     print(json.dumps(res))
 
 
+def seq_bench(args):
+    """The headline: a sequence of 257^3 stretched-grid Poisson solves whose right-hand side
+    evolves smoothly in time (the seeded Fourier modes of the single-solve workload with amplitudes
+    a_m cos(w_m t + psi_m), dt 0.05), as a time stepper produces them.  Every solve runs to the same
+    stop rule (relative residual 1e-10 on the projected right-hand side).
+      learned+mixed: learned initial guess (trained online by its own HIP kernels over the warm-up
+                     solves, then frozen) + FP32 V(2,2) cycle preconditioning GCR(1) under the FP64
+                     residual, exact coarsest level
+      fp64 warm:     plain FP64 V(2,2) multigrid (the reference algorithm, red-black order) from the
+                     previous solution, as hydro_d3 warm-starts mgSolve
+    A step's time covers the guess and the solve; the right-hand sides are in HBM beforehand."""
+    import numpy as np
+    from saras_amd import sequence, workloads
+    W, K = args.warmup, args.steps
+    n = W + K
+    log = lambda msg: print(f"[seq257] {msg}", file=sys.stderr, flush=True)  # noqa: E731
+    seq = workloads.EvolvingPoisson(ind=8, dt=0.05)
+    log(f"modes ready; uploading {n} right-hand sides")
+    rhs = sequence.upload_rhs(seq, n)
+    res = {}
+    sols = {}
+    legs = (("learned_mixed", "learned_gcr", 1), ("fp64_warm", "fp64_warm", 0), ("fp64_zero", "fp64_zero", 0),
+            ("mixed_refinement_warm", "mixed_warm", 0), ("gcr_warm", "gcr_warm", 1), ("gcr_extrap", "gcr_extrap", 1))
+    net = None
+    for name, var, direct in legs:
+        log(f"leg {name}")
+        recs, net_v, last = sequence.run(seq, var, n, ntrain=W if var.startswith("learned") else 0, rhs=rhs,
+                                         coarse_direct=direct)
+        s = sequence.summary(recs, W)
+        s["cycles_per_step"] = [r["cycles"] for r in recs[W:]]
+        if var.startswith("learned"):
+            losses = [r["train_loss"] for r in recs if r.get("train_loss") is not None]
+            s["train_loss_first_last"] = [losses[0], losses[-1]] if losses else None
+            s["kappa"] = net_v.kappa()[0]
+            net = net_v
+        s["sweep0_ms_mean"] = float(np.mean([r["sweep0_ms"] for r in recs[W:]]))
+        res[name] = s
+        sols[name] = last
+    if net is not None:
+        net.close()
+    for d in rhs:
+        d.free()
+    a, b = sols["learned_mixed"][1:-1, 1:-1, 1:-1], sols["fp64_warm"][1:-1, 1:-1, 1:-1]
+    dm = (a - a.mean()) - (b - b.mean())
+    lm, base = res["learned_mixed"], res["fp64_warm"]
+    # roofline of the dominant kernel of the learned+mixed solve: the finest FP32 half sweep, its
+    # average launch time from HIP events on the solver's stream over the timed solves
+    from saras_amd import mg
+    p = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)[0]
+    p.precision, p.coarseDirect = mg._L.SR_PREC_MIXED_GCR, 1
+    solver = mg.Multigrid(seq.g, p)
+    sweep_ms = lm["sweep0_ms_mean"]
+    bytes_sweep = solver.kernel_bytes(6, 0)
+    iso_ms = solver.time_kernel(6, 0, 20)
+    solver.close()
+    achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else None
+    traffic = TRAFFIC_BYTES.get("k_rb_sweep<float> 257^3")
+    cpu = None
+    if not args.no_cpu:
+        pc, gc, fc = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)
+        cpu_ms, cpu_cycles, cpu_res = cpu_baseline(pc, gc, seq.rhs(W), args.cpu_cycles)
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+               "sample": f"one solve (right-hand side of step {W}, zero guess) of the same 257^3 sequence by the "
+                         f"oracle's restatement of the reference algorithm (lexicographic GS V(2,2), -Ofast, "
+                         f"1 thread): {cpu_cycles} cycles to rel. residual {cpu_res:.2e}"}
+    out = {
+        "metric": METRIC, "value": round(lm["ms_per_solve"], 4), "unit": "ms/solve", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(lm["ms_per_solve"], 4), "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64+f32+bf16", "data": "synthetic",
+        "config": {"workload": "sequence of 257^3 stretched-grid Poisson solves (PPN, tanh z beta=1, 8 seeded "
+                               "Fourier modes with amplitudes a_m cos(w_m t + psi_m), dt=0.05), each to 1e-10 rel. "
+                               "residual; learned guess trained online over the warm-up solves",
+                   "nodes": [257, 257, 257], "vcDepth": seq.p.vcDepth,
+                   "speedup_vs_fp64_warm": round(base["ms_per_solve"] / lm["ms_per_solve"], 3),
+                   "speedup_vs_fp64_zero": round(res["fp64_zero"]["ms_per_solve"] / lm["ms_per_solve"], 3),
+                   "learned_mixed_max_abs_diff_vs_fp64_last_step": float(np.max(np.abs(dm))),
+                   "learned_mixed_rel_l2_diff_vs_fp64_last_step": float(np.linalg.norm(dm) /
+                                                                       np.linalg.norm(b - b.mean())),
+                   "legs": res},
+        "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float> finest level (FP32 V-cycle)",
+                     "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
+                     "bytes_per_launch": bytes_sweep, "avg_launch_ms_in_solve": round(sweep_ms, 5),
+                     "avg_launch_ms_isolated": round(iso_ms, 5)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps (default 30 for the sequence and convection workloads: the learned "
+                         "guess trains over them; 2 otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cycles", type=int, default=60)
-    ap.add_argument("--workload", default="poisson257", choices=["poisson257", "tgv129", "rbc257", "rbc257x129"])
+    ap.add_argument("--workload", default="seq257", choices=["seq257", "poisson257", "tgv129", "rbc257", "rbc257x129"])
     ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the solver on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.warmup is None:
+        args.warmup = 30 if args.workload in ("seq257", "rbc257", "rbc257x129") else 2
+    if args.workload == "seq257":
+        seq_bench(args)
+        return
     if args.workload == "tgv129":
         tgv_bench(args)
         return
@@ -283,7 +391,8 @@ def main():
                    "solve_GBs_algorithmic": round(solve_gbs, 1), "mixed_precision": mixed, "plan": plan},
         "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<double> finest level",
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": TRAFFIC_BYTES.get("k_rb_sweep<double> 257^3"),
                      "bytes_per_launch": bytes_sweep, "avg_launch_ms_in_solve": round(sweep_ms, 5),
                      "avg_launch_ms_isolated": round(iso_ms, 5)},
         "cpu_baseline": cpu,

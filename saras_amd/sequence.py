@@ -43,12 +43,25 @@ def net_grid_for(shape):
     return tuple((k - 2 - 1) // 2 + 1 for k in shape)
 
 
-def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None):
+def upload_rhs(seq, nsteps):
+    """The first `nsteps` right-hand sides of `seq`, each in its own HBM buffer."""
+    out = []
+    for n in range(nsteps):
+        d = _L.DeviceArray(seq.shape)
+        d.upload(seq.rhs(n))
+        out.append(d)
+    return out
+
+
+def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None, rhs=None, coarse_direct=0):
     """Run `nsteps` solves of the sequence `seq` (workloads.EvolvingPoisson).  Returns a list of
-    per-step records and the network handle used (if any)."""
+    per-step records and the network handle used (if any).  `rhs`: optional list of device buffers
+    holding the right-hand sides (upload_rhs); otherwise each is uploaded before its step.
+    coarse_direct: solve the coarsest level exactly (sr_mg_desc.coarseDirect)."""
     assert variant in VARIANTS, variant
     p = copy.copy(seq.p)
     p.precision = precision_of(variant)
+    p.coarseDirect = int(coarse_direct)
     solver = mg.Multigrid(seq.g, p)
     shape = seq.shape
     d_f = _L.DeviceArray(shape)
@@ -62,7 +75,11 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None)
         net = GuessNet(net_grid_for(shape), seed=seed, lr=lr, fine=tuple(k - 2 for k in shape))
     recs = []
     for n in range(nsteps):
-        d_f.upload(seq.rhs(n))
+        if rhs is None:
+            d_f.upload(seq.rhs(n))
+            f_ptr = d_f.ptr
+        else:
+            f_ptr = rhs[n].ptr
         cur = ring[n % 3]
         p1 = ring[(n - 1) % 3] if n >= 1 else None
         p2 = ring[(n - 2) % 3] if n >= 2 else None
@@ -75,14 +92,14 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None)
         elif variant.endswith("extrap"):
             net.extrapolate(p1.ptr, p2.ptr if p2 else None, cur.ptr)
         else:
-            st = net.predict(solver, p1.ptr, p2.ptr if p2 else None, d_f.ptr, cur.ptr)
+            st = net.predict(solver, p1.ptr, p2.ptr if p2 else None, f_ptr, cur.ptr)
             rec.update(choice=st.choice, rel_warm=st.rel_warm, rel_extrap=st.rel_extrap,
                        rel_learned=st.rel_learned, predict_ms=st.predict_ms)
         t1 = time.perf_counter()
-        stats = solver.mgSolve_device(cur.ptr, d_f.ptr)
+        stats = solver.mgSolve_device(cur.ptr, f_ptr)
         t2 = time.perf_counter()
         rec.update(ms=(t2 - t0) * 1e3, guess_ms=(t1 - t0) * 1e3, solve_ms=(t2 - t1) * 1e3, cycles=stats.cycles,
-                   converged=bool(stats.converged), residual=stats.residual)
+                   converged=bool(stats.converged), residual=stats.residual, sweep0_ms=stats.sweep0_ms)
         hist = solver.history()
         rec["first_cycle_residual"] = float(hist[0]) if len(hist) else None
         if variant.startswith("learned") and p1 is not None and n < ntrain:
