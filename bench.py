@@ -135,7 +135,7 @@ def rbc_bench(args):
               Re=1.0, cn_tol=1e-6, semantics=_lib.SR_SEM_PROJECTED, rel_tol=1e-10, vc_count=100, pre=2, post=2,
               res_type=2, depth=nz_ind - 1)
     n = (257, 257, 257 if nz_ind == 8 else 129)
-    net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3)
+    net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3, history=8)
     run = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1, **kw))
     run.attach_guess(net, train=True)
     run.initRBC(seed=0, amp=1e-3)
@@ -147,14 +147,15 @@ def rbc_bench(args):
             train_loss.append(st.train_loss)
     train_s = time.perf_counter() - t0
     snap = {w: run.download(w) for w in (ts.VX, ts.VY, ts.VZ, ts.P, ts.PP, ts.T)}
-    # (a) learned + mixed, frozen network, from the trained state
+    # (a) learned + mixed, frozen network (used adaptively), from the trained state
     run.attach_guess(net, train=False)
+    net.set_policy(2)
     rec_l = [run.timeAdvance() for _ in range(args.steps)]
     learned = {"ms_per_solve": float(np.mean([r.solve_ms + r.guess_ms for r in rec_l])),
                "cycles_mean": float(np.mean([r.mg_cycles for r in rec_l])),
                "guess_ms_mean": float(np.mean([r.guess_ms for r in rec_l])),
-               "learned_chosen": int(sum(1 for r in rec_l if r.guess_choice == 2)),
-               "extrap_chosen": int(sum(1 for r in rec_l if r.guess_choice == 1)),
+               "choices": {name: int(sum(1 for r in rec_l if r.guess_choice == c))
+                           for c, name in ((0, "warm"), (1, "extrapolation"), (2, "learned"), (3, "linear"))},
                "rel_guess_median": float(np.median([r.rel_guess for r in rec_l])),
                "step_ms_mean": float(np.mean([r.step_ms for r in rec_l]))}
     nu_end = run.nusselt()
@@ -215,25 +216,26 @@ def seq_bench(args):
     rhs = sequence.upload_rhs(seq, n)
     res = {}
     sols = {}
-    legs = (("learned_mixed", "learned_gcr", 1), ("fp64_warm", "fp64_warm", 0), ("fp64_zero", "fp64_zero", 0),
-            ("mixed_refinement_warm", "mixed_warm", 0), ("gcr_warm", "gcr_warm", 1), ("gcr_extrap", "gcr_extrap", 1))
-    net = None
-    for name, var, direct in legs:
+    # (leg, sequence variant, exact coarsest level, history of the linear predictor)
+    legs = (("learned_mixed", "learned_gcr", 1, 8), ("fp64_warm", "fp64_warm", 0, 0), ("fp64_zero", "fp64_zero", 0, 0),
+            ("mixed_refinement_warm", "mixed_warm", 0, 0), ("gcr_warm", "gcr_warm", 1, 0),
+            ("gcr_extrap", "gcr_extrap", 1, 0), ("linear_predictor_gcr", "linear_gcr", 1, 8),
+            ("learned_no_history_gcr", "learned_gcr", 1, 0))
+    for name, var, direct, hist in legs:
         log(f"leg {name}")
         recs, net_v, last = sequence.run(seq, var, n, ntrain=W if var.startswith("learned") else 0, rhs=rhs,
-                                         coarse_direct=direct)
+                                         coarse_direct=direct, history=hist)
         s = sequence.summary(recs, W)
         s["cycles_per_step"] = [r["cycles"] for r in recs[W:]]
         if var.startswith("learned"):
-            losses = [r["train_loss"] for r in recs if r.get("train_loss") is not None]
+            losses = [r["train_loss"] for r in recs if r.get("train_loss") is not None and r["train_loss"] >= 0]
             s["train_loss_first_last"] = [losses[0], losses[-1]] if losses else None
             s["kappa"] = net_v.kappa()[0]
-            net = net_v
+        if net_v is not None:
+            net_v.close()
         s["sweep0_ms_mean"] = float(np.mean([r["sweep0_ms"] for r in recs[W:]]))
         res[name] = s
         sols[name] = last
-    if net is not None:
-        net.close()
     for d in rhs:
         d.free()
     a, b = sols["learned_mixed"][1:-1, 1:-1, 1:-1], sols["fp64_warm"][1:-1, 1:-1, 1:-1]

@@ -33,7 +33,11 @@ typedef struct sr_guess_desc {
     int fine[3];                  /* solver grid nodes: n (or 0) or 2n - 1 in every direction  */
     unsigned long long seed;      /* initial weights (He normal, zero biases)                   */
     float lr, beta1, beta2, eps;  /* Adam                                                        */
+    int history;                  /* 0..SR_GUESS_MAX_HISTORY: converged (rhs, solution) pairs kept
+                                     for the linear least-squares predictor (0: off)              */
 } sr_guess_desc;
+
+#define SR_GUESS_MAX_HISTORY 8
 
 int sr_guess_create(const sr_guess_desc *desc, sr_guess **out);
 int sr_guess_destroy(sr_guess *g);
@@ -57,27 +61,43 @@ int sr_guess_tensor(sr_guess *g, int which, float *host, long *count);
 
 /* ---- use inside a sequence of pressure solves --------------------------------------------- */
 typedef struct sr_guess_stats {
-    double rel_warm;              /* relative residual (sr_mg_residual) of the warm start p1     */
-    double rel_extrap;            /* ... of the extrapolation 2 p1 - p2                           */
-    double rel_learned;           /* ... of the extrapolation plus the network's correction      */
-    int choice;                   /* guess written to p_out: 0 warm start, 1 extrapolation, 2 learned */
+    double rel_warm;              /* relative residual (sr_mg_residual) of the warm start p1, or -1 */
+    double rel_extrap;            /* ... of the extrapolation 2 p1 - p2, or -1                    */
+    double rel_learned;           /* ... of the base guess plus the network's correction, or -1  */
+    int choice;                   /* guess written to p_out: 0 warm start, 1 extrapolation,
+                                     2 learned (base + network correction), 3 linear predictor   */
     double predict_ms;
+    double rel_linear;            /* relative residual of the linear predictor (from its
+                                     least-squares fit; exact when the network ran), or -1      */
+    int net_used;                 /* 1 when the network ran for this guess                       */
 } sr_guess_stats;
 
 /* Initial guess for solving L p = rhs with solver m (whose grid is the handle's `fine` grid):
  * p1, p2 = the two previous solutions (p2 may be NULL), all device pointers in the padded layout.
- * Writes the best of {p1, 2 p1 - p2, 2 p1 - p2 + learned correction} by relative residual into
- * p_out.  Keeps the network's forward pass for a following sr_guess_train.                      */
+ * Base guess: with a history (desc.history > 0, projected semantics, at least one pair observed)
+ * the linear predictor -- the least-squares combination of the observed solutions whose right-hand
+ * sides best fit rhs (Gram-Schmidt basis of the projected right-hand sides; its residual is known
+ * from the fit) -- which contains p1 and 2 p1 - p2 in its span; otherwise 2 p1 - p2 (or p1 when
+ * its residual is lower).  The network's correction is added to the base and kept only when its
+ * relative residual (sr_mg_residual) is lower.  Keeps the forward pass for sr_guess_train.       */
 int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const double *p2, const double *rhs,
                      double *p_out, sr_guess_stats *stats);
 /* One training step on the converged solution p_star of the solve the last predict was for:
- * target = p_star - (2 p1 - p2) restricted to the network grid; MSE loss, backward pass, Adam.
- * *rel_loss = ||y - target||^2 / ||target||^2 (normalised units) before the step.               */
+ * target = p_star - (base guess) restricted to the network grid; MSE loss, backward pass, Adam.
+ * *rel_loss = ||y - target||^2 / ||target||^2 (normalised units) before the step; -1 when the
+ * last predict did not run the network (nothing to learn from).                                 */
 int sr_guess_train(sr_guess *g, const double *p_star, double *rel_loss);
+/* Add the converged pair (rhs, p_star) of the solve just finished to the linear predictor's
+ * history (call after every solve, trained or not; a no-op without a history).  When the history
+ * is full it keeps the newest half of its basis and adds this pair.                            */
+int sr_guess_observe(sr_guess *g, sr_mg *m, const double *rhs, const double *p_star);
 /* p_out = 2 p1 - p2 (p1 when p2 is NULL): the extrapolated guess alone, for comparisons */
 int sr_guess_extrapolate(sr_guess *g, const double *p1, const double *p2, double *p_out);
-/* policy 0 (default): sr_guess_predict returns the best of the three guesses by relative
- * residual; 1: always the learned one (ablation and diagnostics)                               */
+/* policy 0 (default): sr_guess_predict runs the network every time and returns the best guess by
+ * relative residual; 1: always the learned one (ablation and diagnostics); 2: adaptive -- after
+ * three predictions in a row where the network's guess lost, the network is skipped for the next
+ * eight (for a frozen network; training needs policy 0); 3: never the network (the base guess
+ * alone: linear predictor or extrapolation; ablation)                                           */
 int sr_guess_set_policy(sr_guess *g, int policy);
 /* the correction scale kappa and the number of training steps taken */
 int sr_guess_kappa(const sr_guess *g, double *kappa, int *trained);

@@ -72,12 +72,12 @@ _fp = C.POINTER(C.c_float)
 
 class SrGuessDesc(C.Structure):
     _fields_ = [("n", C.c_int * 3), ("fine", C.c_int * 3), ("seed", C.c_ulonglong), ("lr", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("history", C.c_int)]
 
 
 class SrGuessStats(C.Structure):
     _fields_ = [("rel_warm", C.c_double), ("rel_extrap", C.c_double), ("rel_learned", C.c_double),
-                ("choice", C.c_int), ("predict_ms", C.c_double)]
+                ("choice", C.c_int), ("predict_ms", C.c_double), ("rel_linear", C.c_double), ("net_used", C.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/*.h
@@ -145,6 +145,7 @@ SIGNATURES = [
     ("sr_guess_predict", C.c_int, [_vp, _vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(SrGuessStats)]),
     ("sr_guess_train", C.c_int, [_vp, C.c_void_p, _dp]),
+    ("sr_guess_observe", C.c_int, [_vp, _vp, C.c_void_p, C.c_void_p]),
     ("sr_guess_kappa", C.c_int, [_vp, _dp, _ip]),
     ("sr_guess_extrapolate", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sr_guess_set_policy", C.c_int, [_vp, C.c_int]),

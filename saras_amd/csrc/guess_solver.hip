@@ -1,13 +1,15 @@
 // Solver-facing entry points of the learned initial guess (include/saras_guess.h): the guess for
 // the next pressure solve and its online training on the converged solution.
 //
-// Guess = extrapolation 2 p1 - p2 of the previous two solutions plus a correction predicted by the
-// network from the extrapolation's residual r = f - L(2 p1 - p2), the last increment p1 - p2 and
-// f (each restricted to the network grid and scaled to unit RMS); the correction is the network
-// output times kappa * RMS(r), prolonged to the solver grid.  kappa is the running ratio of the
-// RMS of the true correction to the RMS of r, measured during training.  Before a guess is
-// returned its relative residual is compared with those of the extrapolation and of the warm start
-// p1; the best of the three is used.
+// Guess = a base guess plus a correction predicted by the network from the base's residual
+// r = f - L(base), the last increment p1 - p2 and f (each restricted to the network grid and
+// scaled to unit RMS); the correction is the network output times kappa * RMS(r), prolonged to the
+// solver grid.  kappa is the running ratio of the RMS of the true correction to the RMS of r,
+// measured during training.  The base is the linear predictor when a history is kept (the
+// least-squares fit of the new right-hand side by the previous, projected ones, applied to their
+// solutions: a Gram-Schmidt basis q_i of right-hand sides with L x_i = q_i, so the fit's residual
+// is known without a stencil pass), otherwise the extrapolation 2 p1 - p2.  The returned guess is
+// the one with the lowest relative residual.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -267,9 +269,252 @@ __global__ void __launch_bounds__(256) k_loss(long n, const float *__restrict__ 
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// linear predictor: projection of the right-hand side, inner products with the basis, linear
+// combinations; natural padded layout, core nodes (i, j, k) in [0, n)
+// ------------------------------------------------------------------------------------------------
+struct Ptr8 {
+    const double *p[SR_GUESS_MAX_HISTORY];
+};
+struct Coef8 {
+    double c[SR_GUESS_MAX_HISTORY];
+};
+
+__device__ __forceinline__ void core_of(const FineDims &f, unsigned t, int &i, int &j, int &k) {
+    unsigned r = t / (unsigned)f.nz;
+    k = (int)(t - r * (unsigned)f.nz);
+    unsigned i0 = r / (unsigned)f.ny;
+    j = (int)(r - i0 * (unsigned)f.ny);
+    i = (int)i0;
+}
+
+// per-block sums of width W (W <= 16) through LDS: part[b * W + c]
+template <int W>
+__device__ inline void block_sums(double (&v)[W], double *part) {
+    __shared__ double red[256];
+    for (int c = 0; c < W; c++) {
+        red[threadIdx.x] = v[c];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(long)blockIdx.x * W + c] = red[0];
+        __syncthreads();
+    }
+}
+
+// sum over the core of w_x(i) w_y(j) w_z(k) rhs (the left-null-vector weight of the projection)
+__global__ void __launch_bounds__(256) k_wsum(FineDims f, const double *__restrict__ rhs, const double *__restrict__ wx,
+                                              const double *__restrict__ wy, const double *__restrict__ wz,
+                                              double *__restrict__ part) {
+    double v[1] = {0.0};
+    unsigned n = (unsigned)f.nx * f.ny * f.nz;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int i, j, k;
+        core_of(f, t, i, j, k);
+        v[0] += wx[i] * wy[j] * wz[k] * rhs[fpad(f, i, j, k)];
+    }
+    block_sums<1>(v, part);
+}
+
+// out = rhs - c on the core (c = sums[0] * scale, or 0 without sums), 0 on the ghosts
+__global__ void __launch_bounds__(256) k_project_rhs(FineDims f, const double *__restrict__ rhs,
+                                                     const double *__restrict__ sums, double scale,
+                                                     double *__restrict__ out) {
+    unsigned n1 = (unsigned)f.ny + 2, n2 = (unsigned)f.nz + 2;
+    unsigned total = ((unsigned)f.nx + 2) * n1 * n2;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    unsigned r = t / n2;
+    int k = (int)(t - r * n2) - 1;
+    unsigned i0 = r / n1;
+    int j = (int)(r - i0 * n1) - 1;
+    int i = (int)i0 - 1;
+    bool core = i >= 0 && j >= 0 && k >= 0 && i < f.nx && j < f.ny && k < f.nz;
+    double c = sums ? sums[0] * scale : 0.0;
+    out[t] = core ? rhs[t] - c : 0.0;
+}
+
+// (b_q, y) for q < nb and (y, y) over the core: part[b * 9 + q], q = nb holds (y, y)
+__global__ void __launch_bounds__(256) k_dots(FineDims f, const double *__restrict__ y, Ptr8 B, int nb,
+                                              double *__restrict__ part) {
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned n = (unsigned)f.nx * f.ny * f.nz;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int i, j, k;
+        core_of(f, t, i, j, k);
+        long q = fpad(f, i, j, k);
+        double yv = y[q];
+#pragma unroll
+        for (int a = 0; a < SR_GUESS_MAX_HISTORY; a++)
+            if (a < nb) v[a] += B.p[a][q] * yv;
+        v[8] += yv * yv;
+    }
+    if (nb < 8) {   // (y, y) goes to slot nb
+        v[nb] = v[8];
+        v[8] = 0.0;
+    }
+    block_sums<9>(v, part);
+}
+
+// out = a0 base + sum_q c_q X_q over every padded entry; with part, per-block sums of out^2 over
+// the core
+__global__ void __launch_bounds__(256) k_lincomb(FineDims f, const double *__restrict__ base, double a0, Ptr8 X,
+                                                 Coef8 c, int nx, double *__restrict__ out, double *__restrict__ part) {
+    unsigned n1 = (unsigned)f.ny + 2, n2 = (unsigned)f.nz + 2;
+    unsigned total = ((unsigned)f.nx + 2) * n1 * n2;
+    double v[1] = {0.0};
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        double s = base ? a0 * base[t] : 0.0;
+#pragma unroll
+        for (int a = 0; a < SR_GUESS_MAX_HISTORY; a++)
+            if (a < nx) s += c.c[a] * X.p[a][t];
+        out[t] = s;
+        if (part) {
+            unsigned r = t / n2;
+            int k = (int)(t - r * n2) - 1;
+            unsigned i0 = r / n1;
+            int j = (int)(r - i0 * n1) - 1;
+            int i = (int)i0 - 1;
+            if (i >= 0 && j >= 0 && k >= 0 && i < f.nx && j < f.ny && k < f.nz) v[0] += s * s;
+        }
+    }
+    if (part) block_sums<1>(v, part);
+}
+
+// out[c] = sum over blocks of part[b * w + c], c < w <= 16: 64 lanes per column
+__global__ void __launch_bounds__(1024) k_sumw_final(const double *__restrict__ part, int nb, int w,
+                                                     double *__restrict__ out) {
+    __shared__ double red[1024];
+    int c = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double s = 0.0;
+    if (c < w)
+        for (int b = l; b < nb; b += 64) s += part[(long)b * w + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 32; h > 0; h >>= 1) {
+        if (l < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (l == 0 && c < w) out[c] = red[threadIdx.x];
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// linear predictor (host side)
+// ------------------------------------------------------------------------------------------------
+static bool linear_on(const sr_guess *g, const sr_mg *m) {
+    return g->hcap > 0 && m->d.semantics == SR_SEM_PROJECTED;
+}
+
+static bool singular_mg(const sr_mg *m) { return !(m->d.testPoisson && !(m->d.xPer && m->d.yPer && m->d.zPer)); }
+
+static Ptr8 ptrs(double *const *a, int n) {
+    Ptr8 P{};
+    for (int i = 0; i < n; i++) P.p[i] = a[i];
+    return P;
+}
+
+// least squares min ||y - sum c_i f_i|| from the Gram matrix G = (f_i, f_j) and d = (f_i, y):
+// c = G^+ d through a Jacobi eigen-decomposition, eigenvalues below 1e-14 of the largest dropped
+// (near-dependent right-hand sides); returns ||y - sum c_i f_i||^2 = yy - 2 c.d + c^T G c
+static double lsq(int n, const double (*Gm)[SR_GUESS_MAX_HISTORY], const double *d, double yy, double *c) {
+    double A[SR_GUESS_MAX_HISTORY][SR_GUESS_MAX_HISTORY], V[SR_GUESS_MAX_HISTORY][SR_GUESS_MAX_HISTORY];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            A[i][j] = Gm[i][j];
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; i++) {
+            diag += A[i][i] * A[i][i];
+            for (int j = i + 1; j < n; j++) off += A[i][j] * A[i][j];
+        }
+        if (off <= 1e-32 * diag) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                if (A[p][q] == 0.0) continue;
+                double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < n; k++) {
+                    double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = cs * akp - sn * akq;
+                    A[k][q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = cs * apk - sn * aqk;
+                    A[q][k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = cs * vkp - sn * vkq;
+                    V[k][q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    double lmax = 0.0;
+    for (int k = 0; k < n; k++) lmax = std::max(lmax, A[k][k]);
+    for (int i = 0; i < n; i++) c[i] = 0.0;
+    for (int k = 0; k < n; k++) {
+        if (!(A[k][k] > 1e-14 * lmax)) continue;
+        double vd = 0.0;
+        for (int i = 0; i < n; i++) vd += V[i][k] * d[i];
+        for (int i = 0; i < n; i++) c[i] += V[i][k] * vd / A[k][k];
+    }
+    double r2 = yy;
+    for (int i = 0; i < n; i++) {
+        r2 -= 2.0 * c[i] * d[i];
+        for (int j = 0; j < n; j++) r2 += c[i] * Gm[i][j] * c[j];
+    }
+    return std::max(r2, 0.0);
+}
+
+// fcur = the right-hand side projected like the solver projects it; hd = (f_i, fcur), yy, and the
+// least-squares coefficients hc with the fit's residual
+static int lin_fit(sr_guess *g, sr_mg *m, const double *rhs) {
+    hipStream_t s = g->s;
+    const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
+    if (singular_mg(m)) {
+        hipLaunchKernelGGL(k_wsum, dim3(kRedBlocks), dim3(256), 0, s, f, rhs, m->wvec[0], m->wvec[1], m->wvec[2],
+                           g->lpart);
+        hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kRedBlocks, 1, g->lsum);
+        hipLaunchKernelGGL(k_project_rhs, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, rhs, g->lsum, 1.0 / m->wsum,
+                           g->fcur);
+    } else {
+        hipLaunchKernelGGL(k_project_rhs, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, rhs, (const double *)nullptr,
+                           0.0, g->fcur);
+    }
+    hipLaunchKernelGGL(k_dots, dim3(kRedBlocks), dim3(256), 0, s, f, g->fcur, ptrs(g->hf, g->hn), g->hn, g->lpart);
+    hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kRedBlocks, 9, g->lsum);
+    HIPCHK(hipMemcpyAsync(g->lsum_h, g->lsum, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < g->hn; i++) g->hd[i] = g->lsum_h[i];
+    g->yy = g->lsum_h[g->hn];
+    g->fit_r2 = g->hn > 0 ? lsq(g->hn, g->G, g->hd, g->yy, g->hc) : g->yy;
+    g->have_fit = true;
+    return SR_OK;
+}
+
+// relative residual of the fit, ||fcur - sum hc_i f_i|| / ||fcur||
+static double lin_rel(const sr_guess *g) { return g->yy > 0.0 ? std::sqrt(g->fit_r2 / g->yy) : 0.0; }
+
+static int lin_apply(sr_guess *g, double *out) {
+    const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
+    Coef8 c{};
+    for (int i = 0; i < g->hn; i++) c.c[i] = g->hc[i];
+    hipLaunchKernelGGL(k_lincomb, dim3(kRedBlocks), dim3(256), 0, g->s, f, (const double *)nullptr, 0.0,
+                       ptrs(g->hp, g->hn), c, g->hn, out, (double *)nullptr);
+    HIPCHK(hipGetLastError());
+    return SR_OK;
+}
 
 // ------------------------------------------------------------------------------------------------
 // entry points
@@ -284,65 +529,129 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
     hipStream_t s = g->s;
     const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
     const NetGrid &n0 = g->lv[0];
-    // baselines: the warm start p1 and the extrapolation
-    hipLaunchKernelGGL(k_extrap, dim3(nblk(g->fine_pad)), dim3(256), 0, s, g->fine_pad, p1, p2, g->extrap);
-    HIPCHK(hipStreamSynchronize(s));
-    double rel_ex = 0.0, rel_warm = 0.0, rel_learned = 0.0;
-    RETCHK(sr_mg_residual(m, g->extrap, rhs, g->rfield, &rel_ex));
-    if (p2) RETCHK(sr_mg_residual(m, p1, rhs, nullptr, &rel_warm));
-    else rel_warm = rel_ex;
-    // network input, forward pass, correction
-    hipLaunchKernelGGL(k_gather, dim3(nblk(n0.nvox())), dim3(256), 0, s, f, n0, g->ratio, g->rfield, p1, p2, rhs,
-                       g->stage4);
-    hipLaunchKernelGGL(k_sumsq_part, dim3(kRedBlocks), dim3(256), 0, s, n0.nvox(), g->stage4, g->rpart);
-    hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(256), 0, s, g->rpart, kRedBlocks, g->dsc);
-    hipLaunchKernelGGL(k_make_input, dim3(nblk(n0.nvox() * SR_GUESS_IN_CH)), dim3(256), 0, s, n0.nvox(), g->stage4,
-                       g->dsc, g->T[X0]);
-    RETCHK(guess_forward_device(g));
-    hipLaunchKernelGGL(k_apply, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, n0, g->ratio, g->extrap, g->y, g->dsc,
-                       g->kappa, p_out);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));
-    g->have_cache = true;
-    int choice = 2;
-    if (g->kappa > 0.0) {
-        RETCHK(sr_mg_residual(m, p_out, rhs, nullptr, &rel_learned));
+    double rel_ex = -1.0, rel_warm = -1.0, rel_learned = -1.0, rel_lin = -1.0;
+    // base guess: the linear predictor when it has a history, else the extrapolation
+    const double *base = g->extrap;
+    int base_choice = 1;
+    double rel_base = 0.0;
+    bool rfield_ok = false;
+    g->have_fit = false;
+    if (linear_on(g, m)) RETCHK(lin_fit(g, m, rhs));
+    if (linear_on(g, m) && g->hn >= 1) {
+        RETCHK(lin_apply(g, g->xlin));
+        rel_lin = lin_rel(g);
+        base = g->xlin;
+        base_choice = 3;
+        rel_base = rel_lin;
     } else {
-        rel_learned = rel_ex;   // untrained: the correction is zero, the guess is the extrapolation
-        choice = 1;
+        hipLaunchKernelGGL(k_extrap, dim3(nblk(g->fine_pad)), dim3(256), 0, s, g->fine_pad, p1, p2, g->extrap);
+        HIPCHK(hipStreamSynchronize(s));
+        RETCHK(sr_mg_residual(m, g->extrap, rhs, g->rfield, &rel_ex));
+        rfield_ok = true;
+        if (p2) RETCHK(sr_mg_residual(m, p1, rhs, nullptr, &rel_warm));
+        rel_base = rel_ex;
     }
-    if (g->policy == 1) {
-        if (g->kappa <= 0.0) RETCHK(sr_mg_residual(m, p_out, rhs, nullptr, &rel_learned));
-        choice = 2;
-    } else {
-        if (choice == 2 && !(rel_learned < rel_ex && rel_learned < rel_warm)) choice = rel_ex <= rel_warm ? 1 : 0;
-        if (choice == 1 && rel_warm < rel_ex) choice = 0;
+    // the network's correction on top of the base
+    bool use_net = g->policy != 3;
+    if (g->policy == 2 && g->net_skip > 0) {
+        g->net_skip -= 1;
+        use_net = false;
+    }
+    int choice = base_choice;
+    g->have_cache = false;
+    if (use_net) {
+        HIPCHK(hipStreamSynchronize(s));
+        if (!rfield_ok) RETCHK(sr_mg_residual(m, base, rhs, g->rfield, &rel_base));
+        if (base_choice == 3) rel_lin = rel_base;   // exact now
+        hipLaunchKernelGGL(k_gather, dim3(nblk(n0.nvox())), dim3(256), 0, s, f, n0, g->ratio, g->rfield, p1, p2, rhs,
+                           g->stage4);
+        hipLaunchKernelGGL(k_sumsq_part, dim3(kRedBlocks), dim3(256), 0, s, n0.nvox(), g->stage4, g->rpart);
+        hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(256), 0, s, g->rpart, kRedBlocks, g->dsc);
+        hipLaunchKernelGGL(k_make_input, dim3(nblk(n0.nvox() * SR_GUESS_IN_CH)), dim3(256), 0, s, n0.nvox(),
+                           g->stage4, g->dsc, g->T[X0]);
+        RETCHK(guess_forward_device(g));
+        hipLaunchKernelGGL(k_apply, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, n0, g->ratio, base, g->y, g->dsc,
+                           g->kappa, p_out);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        g->have_cache = true;
+        g->train_base = base;
+        if (g->kappa > 0.0 || g->policy == 1) {
+            RETCHK(sr_mg_residual(m, p_out, rhs, nullptr, &rel_learned));
+            if (g->policy == 1 || rel_learned < rel_base) choice = 2;
+        }
+        if (g->policy == 2) {
+            if (choice == 2) {
+                g->net_losses = 0;
+            } else if (++g->net_losses >= 3) {
+                g->net_losses = 0;
+                g->net_skip = 8;
+            }
+        }
+    }
+    // without a history the warm start competes too
+    if (rel_warm >= 0.0 && g->policy != 1) {
+        double best = choice == 2 ? rel_learned : rel_ex;
+        if (rel_warm < best) choice = 0;
     }
     if (choice != 2) {
-        const double *src = choice == 1 ? g->extrap : p1;
+        const double *src = choice == 0 ? p1 : base;
         HIPCHK(hipMemcpyAsync(p_out, src, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
     }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
         st->rel_warm = rel_warm;
         st->rel_extrap = rel_ex;
         st->rel_learned = rel_learned;
+        st->rel_linear = rel_lin;
         st->choice = choice;
+        st->net_used = use_net ? 1 : 0;
         st->predict_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
     return SR_OK;
 }
 
+extern "C" int sr_guess_observe(sr_guess *g, sr_mg *m, const double *rhs, const double *p_star) {
+    if (!g || !m || !rhs || !p_star) return set_err(SR_ERR_ARG, "null argument");
+    if (!linear_on(g, m)) return SR_OK;
+    const Lay &L = m->lv[0].L;
+    if (L.nx != g->fine[0] || L.ny != g->fine[1] || L.nz != g->fine[2])
+        return set_err(SR_ERR_ARG, "solver grid differs from the guess handle's solver grid");
+    if (!g->have_fit) RETCHK(lin_fit(g, m, rhs));
+    g->have_fit = false;
+    if (!(g->yy > 0.0)) return SR_OK;   // a zero right-hand side adds nothing
+    // the pair goes to the next ring slot (the oldest once the ring is full)
+    int slot = g->hnext;
+    hipStream_t s = g->s;
+    HIPCHK(hipMemcpyAsync(g->hf[slot], g->fcur, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g->hp[slot], p_star, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
+    int n_new = std::min(g->hn + 1, g->hcap);
+    for (int i = 0; i < n_new; i++) {
+        if (i == slot) continue;
+        g->G[slot][i] = g->G[i][slot] = g->hd[i];
+    }
+    g->G[slot][slot] = g->yy;
+    g->hn = n_new;
+    g->hnext = (slot + 1) % g->hcap;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return SR_OK;
+}
+
 extern "C" int sr_guess_train(sr_guess *g, const double *p_star, double *rel_loss) {
     if (!g || !p_star) return set_err(SR_ERR_ARG, "null argument");
-    if (!g->have_cache) return set_err(SR_ERR_STATE, "train needs the forward pass of a preceding predict");
+    if (!g->have_cache) {   // the last predict skipped the network: nothing to learn from
+        if (rel_loss) *rel_loss = -1.0;
+        return SR_OK;
+    }
     hipStream_t s = g->s;
     const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
     const NetGrid &n0 = g->lv[0];
     const long n = n0.nvox();
     // true correction on the network grid and its RMS: calibrates kappa
-    hipLaunchKernelGGL(k_delta, dim3(kRedBlocks), dim3(256), 0, s, f, n0, g->ratio, p_star, g->extrap, g->y,
+    hipLaunchKernelGGL(k_delta, dim3(kRedBlocks), dim3(256), 0, s, f, n0, g->ratio, p_star, g->train_base, g->y,
                        g->target, g->rpart);
     hipLaunchKernelGGL(k_sum4_final, dim3(1), dim3(256), 0, s, g->rpart, kRedBlocks, g->dsc + 4);
     HIPCHK(hipMemcpyAsync(g->hsc, g->dsc, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
@@ -378,8 +687,10 @@ extern "C" int sr_guess_extrapolate(sr_guess *g, const double *p1, const double 
 }
 
 extern "C" int sr_guess_set_policy(sr_guess *g, int policy) {
-    if (!g || policy < 0 || policy > 1) return set_err(SR_ERR_ARG, "policy must be 0 or 1");
+    if (!g || policy < 0 || policy > 3) return set_err(SR_ERR_ARG, "policy must be 0..3");
     g->policy = policy;
+    g->net_losses = 0;
+    g->net_skip = 0;
     return SR_OK;
 }
 

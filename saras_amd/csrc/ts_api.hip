@@ -638,14 +638,17 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
         RETCHK(sr_guess_predict(t->guess, t->mg, t->Pp1, t->nsolved >= 2 ? t->Pp2 : nullptr, t->mgRHS, t->Pp, &gs));
         choice = gs.choice;
         guess_ms = gs.predict_ms;
-        rel_guess = choice == 2 ? gs.rel_learned : choice == 1 ? gs.rel_extrap : gs.rel_warm;
+        rel_guess = choice == 3 ? gs.rel_linear : choice == 2 ? gs.rel_learned : choice == 1 ? gs.rel_extrap : gs.rel_warm;
     }
     sr_mg_stats ms;
     auto ts0 = std::chrono::steady_clock::now();
     RETCHK(sr_mg_solve(t->mg, t->Pp, t->mgRHS, 1, &ms));
     auto ts1 = std::chrono::steady_clock::now();
     t->mg_cycles = ms.cycles;
-    if (t->guess && choice >= 0 && t->train) RETCHK(sr_guess_train(t->guess, t->Pp, &loss));
+    if (t->guess) {
+        if (choice >= 0 && t->train) RETCHK(sr_guess_train(t->guess, t->Pp, &loss));
+        RETCHK(sr_guess_observe(t->guess, t->mg, t->mgRHS, t->Pp));
+    }
     ts_ghost_wave(t->fp, t->Pp, t->sync_p, t->s);
     if (t->guess) {
         std::swap(t->Pp1, t->Pp2);

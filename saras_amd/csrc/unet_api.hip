@@ -115,6 +115,20 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
     RETCHK(dalloc(g, &g->rpart, 4096L * 4));
     RETCHK(dalloc(g, &g->dsc, 16));
     HIPCHK(hipHostMalloc((void **)&g->hsc, sizeof(double) * 16, hipHostMallocDefault));
+    if (desc->history < 0 || desc->history > SR_GUESS_MAX_HISTORY)
+        return set_err(SR_ERR_ARG, "history must be 0..SR_GUESS_MAX_HISTORY");
+    g->hcap = desc->history;
+    if (g->hcap > 0) {
+        for (int i = 0; i < g->hcap; i++) {
+            RETCHK(dalloc(g, &g->hf[i], g->fine_pad));
+            RETCHK(dalloc(g, &g->hp[i], g->fine_pad));
+        }
+        RETCHK(dalloc(g, &g->fcur, g->fine_pad));
+        RETCHK(dalloc(g, &g->xlin, g->fine_pad));
+        RETCHK(dalloc(g, &g->lpart, 1024L * 16));
+        RETCHK(dalloc(g, &g->lsum, 16));
+        HIPCHK(hipHostMalloc((void **)&g->lsum_h, sizeof(double) * 16, hipHostMallocDefault));
+    }
     // He-normal initial weights (zero biases); the unused input channels and output channels
     // start at zero, the output layer small
     uint64_t st = desc->seed;
@@ -144,6 +158,7 @@ extern "C" int sr_guess_destroy(sr_guess *g) {
     if (g->s) (void)hipStreamSynchronize(g->s);
     for (void *p : g->owned) (void)hipFree(p);
     if (g->hsc) (void)hipHostFree(g->hsc);
+    if (g->lsum_h) (void)hipHostFree(g->lsum_h);
     if (g->s) (void)hipStreamDestroy(g->s);
     delete g;
     return SR_OK;

@@ -51,8 +51,21 @@ struct sr_guess {
     double *rpart = nullptr, *dsc = nullptr, *hsc = nullptr;
     double kappa = 0.0;
     int trained = 0;
-    int policy = 0;   // 0: best of warm start / extrapolation / learned by residual; 1: learned always
+    int policy = 0;   // 0: best guess by residual; 1: learned always; 2: adaptive network use
     bool have_cache = false;
+    const double *train_base = nullptr;   // base guess of the last predict (training target origin)
+    int net_losses = 0, net_skip = 0;     // adaptive policy state
+    // linear predictor: a ring of the last hcap converged pairs (projected right-hand side f_i,
+    // solution p_i, L p_i = f_i to the solve's tolerance) and their Gram matrix (f_i, f_j)
+    int hcap = 0, hn = 0, hnext = 0;
+    double *hf[SR_GUESS_MAX_HISTORY] = {}, *hp[SR_GUESS_MAX_HISTORY] = {};
+    double G[SR_GUESS_MAX_HISTORY][SR_GUESS_MAX_HISTORY] = {};
+    double *fcur = nullptr, *xlin = nullptr;   // projected rhs of the current step, predictor output
+    double hd[SR_GUESS_MAX_HISTORY] = {};      // (f_i, fcur)
+    double hc[SR_GUESS_MAX_HISTORY] = {};      // least-squares coefficients of the current step
+    double yy = 0.0, fit_r2 = 0.0;             // ||fcur||^2 and the fit's squared residual
+    bool have_fit = false;                     // fcur / hd / hc belong to the current step
+    double *lpart = nullptr, *lsum = nullptr, *lsum_h = nullptr;
 };
 
 namespace sr {

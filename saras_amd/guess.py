@@ -25,13 +25,15 @@ def layer_info(layer):
 
 
 class GuessNet:
-    def __init__(self, n, seed=0, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, fine=None):
+    def __init__(self, n, seed=0, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, fine=None, history=0):
+        """history: converged (rhs, solution) pairs kept for the linear predictor (0: off)."""
         self.L = _L.load()
         d = _L.SrGuessDesc()
         d.n[0], d.n[1], d.n[2] = n
         if fine is not None:
             d.fine[0], d.fine[1], d.fine[2] = fine
         d.seed, d.lr, d.beta1, d.beta2, d.eps = seed, lr, beta1, beta2, eps
+        d.history = int(history)
         h = C.c_void_p()
         _L.check(self.L.sr_guess_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -111,10 +113,15 @@ class GuessNet:
         _L.check(self.L.sr_guess_kappa(self.h, C.byref(k), C.byref(t)))
         return k.value, t.value
 
+    def observe(self, solver, rhs_ptr, pstar_ptr):
+        """Add the converged pair of the solve just finished to the linear predictor's history."""
+        _L.check(self.L.sr_guess_observe(self.h, solver.h, C.c_void_p(rhs_ptr), C.c_void_p(pstar_ptr)))
+
     def extrapolate(self, p1_ptr, p2_ptr, out_ptr):
         _L.check(self.L.sr_guess_extrapolate(self.h, C.c_void_p(p1_ptr), C.c_void_p(p2_ptr) if p2_ptr else None,
                                              C.c_void_p(out_ptr)))
 
     def set_policy(self, policy):
-        """0: best of warm start / extrapolation / learned by residual; 1: learned always."""
+        """0: best guess by residual, network every time; 1: learned always; 2: adaptive network use;
+        3: never the network (base guess alone)."""
         _L.check(self.L.sr_guess_set_policy(self.h, int(policy)))

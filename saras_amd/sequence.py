@@ -15,6 +15,7 @@ below differ only in the initial guess handed to the multigrid solve and in its 
                 from the previous solution
   gcr_extrap    the same from 2 p_{n-1} - p_{n-2}
   learned_gcr   the same from the learned guess
+  linear_gcr    the same from the linear predictor alone (the learned guess's base, no network)
 
 Each solve runs to the same stop rule (relative residual 1e-10 on the projected right-hand side);
 the time of a step is the guess plus the solve, with the right-hand side already in HBM."""
@@ -27,7 +28,7 @@ from . import _lib as _L
 from . import mg
 
 VARIANTS = ("fp64_zero", "fp64_warm", "mixed_warm", "mixed_extrap", "learned_mixed", "learned_fp64", "gcr_warm",
-            "gcr_extrap", "learned_gcr")
+            "gcr_extrap", "learned_gcr", "linear_gcr")
 
 
 def precision_of(variant):
@@ -53,11 +54,14 @@ def upload_rhs(seq, nsteps):
     return out
 
 
-def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None, rhs=None, coarse_direct=0):
+def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None, rhs=None, coarse_direct=0,
+        history=0):
     """Run `nsteps` solves of the sequence `seq` (workloads.EvolvingPoisson).  Returns a list of
     per-step records and the network handle used (if any).  `rhs`: optional list of device buffers
     holding the right-hand sides (upload_rhs); otherwise each is uploaded before its step.
-    coarse_direct: solve the coarsest level exactly (sr_mg_desc.coarseDirect)."""
+    coarse_direct: solve the coarsest level exactly (sr_mg_desc.coarseDirect).
+    history: pairs kept by the learned variants' linear predictor (sr_guess_desc.history); after
+    the `ntrain` training solves the network is frozen and used adaptively (policy 2)."""
     assert variant in VARIANTS, variant
     p = copy.copy(seq.p)
     p.precision = precision_of(variant)
@@ -69,10 +73,12 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None,
     d_zero.upload(np.zeros(shape))
     ring = [_L.DeviceArray(shape) for _ in range(3)]
     lib = _L.load()
-    need_net = variant.endswith("extrap") or variant.startswith("learned")
+    need_net = variant.endswith("extrap") or variant.startswith("learned") or variant.startswith("linear")
     if need_net and net is None:
         from .guess import GuessNet
-        net = GuessNet(net_grid_for(shape), seed=seed, lr=lr, fine=tuple(k - 2 for k in shape))
+        net = GuessNet(net_grid_for(shape), seed=seed, lr=lr, fine=tuple(k - 2 for k in shape), history=history)
+        if variant.startswith("linear"):
+            net.set_policy(3)
     recs = []
     for n in range(nsteps):
         if rhs is None:
@@ -84,6 +90,8 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None,
         p1 = ring[(n - 1) % 3] if n >= 1 else None
         p2 = ring[(n - 2) % 3] if n >= 2 else None
         rec = {"step": n}
+        if variant.startswith("learned") and n == ntrain and ntrain > 0:
+            net.set_policy(2)
         t0 = time.perf_counter()
         if variant == "fp64_zero" or p1 is None:
             _L.check(lib.sr_memcpy(cur.ptr, d_zero.ptr, cur.nbytes, 3))
@@ -94,7 +102,8 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None,
         else:
             st = net.predict(solver, p1.ptr, p2.ptr if p2 else None, f_ptr, cur.ptr)
             rec.update(choice=st.choice, rel_warm=st.rel_warm, rel_extrap=st.rel_extrap,
-                       rel_learned=st.rel_learned, predict_ms=st.predict_ms)
+                       rel_learned=st.rel_learned, rel_linear=st.rel_linear, net_used=st.net_used,
+                       predict_ms=st.predict_ms)
         t1 = time.perf_counter()
         stats = solver.mgSolve_device(cur.ptr, f_ptr)
         t2 = time.perf_counter()
@@ -104,6 +113,8 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None,
         rec["first_cycle_residual"] = float(hist[0]) if len(hist) else None
         if variant.startswith("learned") and p1 is not None and n < ntrain:
             rec["train_loss"] = net.train(cur.ptr)
+        if variant.startswith("learned") or variant.startswith("linear"):
+            net.observe(solver, f_ptr, cur.ptr)
         recs.append(rec)
         if on_step:
             on_step(rec)
@@ -123,8 +134,10 @@ def summary(recs, first):
            "max_final_residual": float(max(r["residual"] for r in sel)),
            "guess_ms_mean": float(np.mean([r["guess_ms"] for r in sel]))}
     if "choice" in sel[0]:
-        out["learned_chosen"] = int(sum(1 for r in sel if r.get("choice") == 2))
-        out["rel_extrap_median"] = float(np.median([r["rel_extrap"] for r in sel]))
-        out["rel_learned_median"] = float(np.median([r["rel_learned"] for r in sel]))
-        out["rel_warm_median"] = float(np.median([r["rel_warm"] for r in sel]))
+        out["choices"] = {name: int(sum(1 for r in sel if r.get("choice") == c))
+                          for c, name in ((0, "warm"), (1, "extrapolation"), (2, "learned"), (3, "linear"))}
+        out["net_used"] = int(sum(r.get("net_used", 0) for r in sel))
+        for key in ("rel_extrap", "rel_learned", "rel_warm", "rel_linear"):
+            vals = [r[key] for r in sel if r.get(key, -1.0) >= 0.0]
+            out[key + "_median"] = float(np.median(vals)) if vals else None
     return out

@@ -27,8 +27,9 @@ def test_sequence_learned_guess_converges_and_trains():
     a, b = _centred(sol), _centred(sol_ref)
     assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
     for r in recs[1:]:
-        cands = {0: r["rel_warm"], 1: r["rel_extrap"], 2: r["rel_learned"]}
-        # policy 0: the guess handed to the solve has the smallest relative residual of the three
+        # candidates that were evaluated (-1: not computed for this step)
+        cands = {c: v for c, v in ((0, r["rel_warm"]), (1, r["rel_extrap"]), (2, r["rel_learned"])) if v >= 0.0}
+        # policy 0: the guess handed to the solve has the smallest relative residual of those
         assert cands[r["choice"]] <= min(cands.values()) * (1 + 1e-12)
     losses = [r["train_loss"] for r in recs if r.get("train_loss") is not None]
     assert len(losses) == 23
@@ -38,14 +39,15 @@ def test_sequence_learned_guess_converges_and_trains():
     net.close()
 
 
-def test_time_steps_with_guess_follow_the_plain_run():
+@pytest.mark.parametrize("history", [0, 4])
+def test_time_steps_with_guess_follow_the_plain_run(history):
     from saras_amd import _lib, guess, ts
     kw = dict(ind=(5, 5, 4), L=(2.0, 2.0, 1.0), domain="PPN", mesh="UUD", scalar=True, rbc_type=2, Ra=1e5,
               Pr=1.0, dt=5e-3, prob_type=5, Re=1.0, semantics=_lib.SR_SEM_PROJECTED, rel_tol=1e-11, vc_count=60,
               pre=2, post=2, res_type=2)
     plain = ts.Hydro(ts.ts_desc(**kw))
     learned = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED, **kw))
-    net = guess.GuessNet((17, 17, 9), fine=(33, 33, 17), seed=3)
+    net = guess.GuessNet((17, 17, 9), fine=(33, 33, 17), seed=3, history=history)
     learned.attach_guess(net, train=True)
     for h in (plain, learned):
         h.initRBC(seed=1, amp=0.05)
@@ -56,7 +58,10 @@ def test_time_steps_with_guess_follow_the_plain_run():
         choices.append(st.guess_choice)
         if step >= 1:
             assert st.rel_guess >= 0.0 and st.train_loss >= 0.0
-    assert choices[0] == -1 and all(c in (0, 1, 2) for c in choices[1:])
+    assert choices[0] == -1 and all(c in (0, 1, 2, 3) for c in choices[1:])
+    if history:
+        # from the third step on the linear predictor (or the network on top of it) is the guess
+        assert all(c in (2, 3) for c in choices[2:]), choices
     for which in (ts.VX, ts.VY, ts.VZ, ts.T):
         a, b = learned.download(which), plain.download(which)
         scale = max(float(np.max(np.abs(b))), 1e-300)
