@@ -131,7 +131,8 @@ int sr_mg_op_vcycle(sr_mg *mg);                            /* poisson.cc:219-280
 /* Average device time (ms, HIP events on the solver's stream) of `reps` back-to-back launches of
  * one kernel at one level: kernel 0 = red-black half sweep, 1 = residual, 2 = restriction,
  * 3 = prolongation-correction, 4 = ghost fill, 5 = level-0 residual norm, 6 = the FP32 half
- * sweep of the mixed-precision hierarchy.  Device state is modified (scratch use only).         */
+ * sweep of the mixed-precision hierarchy, 7 / 8 = the single-workgroup coarse end of the FP64 /
+ * FP32 cycle (level argument ignored).  Device state is modified (scratch use only).           */
 int sr_mg_time_kernel(sr_mg *mg, int kernel, int level, int reps, double *avg_ms);
 /* Algorithmic bytes one launch of that kernel moves (the roofline numerator).                   */
 int sr_mg_kernel_bytes(const sr_mg *mg, int kernel, int level, double *bytes);

@@ -434,6 +434,8 @@ extern "C" int sr_mg_kernel_bytes(const sr_mg *m, int kernel, int level, double 
         }
         case 5: *bytes = 2.0 * nn * w; break;                     // norm: read u, f
         case 6: *bytes = 12.0 * nn / 8.0 * 4.0; break;            // FP32 half sweep
+        case 7:
+        case 8: *bytes = 0.0; break;                              // coarse cycle: latency-bound, no HBM figure
         default: return set_err(SR_ERR_ARG, "unknown kernel id");
     }
     return SR_OK;
@@ -458,10 +460,23 @@ extern "C" int sr_mg_time_kernel(sr_mg *m, int kernel, int level, int reps, doub
             case 6:
                 launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], r & 1, bc, false, s);
                 break;
+            case 7: {
+                Level &C = m->lv[m->coarse_first64];
+                launch_coarse_cycle<double>(m->cdesc64, m->clds64, C.u, C.f, m->diters, s);
+                break;
+            }
+            case 8: {
+                Level &C = m->lv[m->coarse_first32];
+                launch_coarse_cycle<float>(m->cdesc32, m->clds32, C.u32, C.f32, m->diters, s);
+                break;
+            }
         }
     };
-    if (kernel < 0 || kernel > 6) return set_err(SR_ERR_ARG, "unknown kernel id");
-    if (kernel == 6 && !V.u32) return set_err(SR_ERR_STATE, "no FP32 hierarchy (mixed precision not enabled)");
+    if (kernel < 0 || kernel > 8) return set_err(SR_ERR_ARG, "unknown kernel id");
+    if ((kernel == 6 || kernel == 8) && !V.u32)
+        return set_err(SR_ERR_STATE, "no FP32 hierarchy (mixed precision not enabled)");
+    if (kernel == 7 && m->coarse_first64 > m->depth) return set_err(SR_ERR_STATE, "no single-workgroup coarse cycle");
+    if (kernel == 8 && m->coarse_first32 > m->depth) return set_err(SR_ERR_STATE, "no single-workgroup coarse cycle");
     launch(0);
     launch(1);
     HIPCHK(hipEventRecord(m->ev0, s));
