@@ -96,8 +96,10 @@ int sr_guess_extrapolate(sr_guess *g, const double *p1, const double *p2, double
 /* policy 0 (default): sr_guess_predict runs the network every time and returns the best guess by
  * relative residual; 1: always the learned one (ablation and diagnostics); 2: adaptive -- after
  * three predictions in a row where the network's guess lost, the network is skipped for the next
- * eight (for a frozen network; training needs policy 0); 3: never the network (the base guess
- * alone: linear predictor or extrapolation; ablation)                                           */
+ * eight, and after each further losing probe for twice as many (up to 256; a win resets it); the
+ * record of policy 0 carries over, so a network that lost its last three comparisons in training
+ * starts skipped (for a frozen network; training needs policy 0); 3: never the network (the base
+ * guess alone: linear predictor or extrapolation; ablation)                                      */
 int sr_guess_set_policy(sr_guess *g, int policy);
 /* the correction scale kappa and the number of training steps taken */
 int sr_guess_kappa(const sr_guess *g, double *kappa, int *trained);

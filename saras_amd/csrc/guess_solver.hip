@@ -580,12 +580,18 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
             RETCHK(sr_mg_residual(m, p_out, rhs, nullptr, &rel_learned));
             if (g->policy == 1 || rel_learned < rel_base) choice = 2;
         }
-        if (g->policy == 2) {
+        if (g->policy == 0 && g->kappa > 0.0) {
+            // keep the record while training so that the adaptive policy starts from it
+            g->net_losses = choice == 2 ? 0 : g->net_losses + 1;
+        } else if (g->policy == 2) {
             if (choice == 2) {
                 g->net_losses = 0;
+                g->net_backoff = 8;
             } else if (++g->net_losses >= 3) {
-                g->net_losses = 0;
-                g->net_skip = 8;
+                // skip for a while, twice as long after every further losing probe (at most 256)
+                g->net_skip = g->net_backoff;
+                g->net_backoff = std::min(2 * g->net_backoff, 256);
+                g->net_losses = 2;
             }
         }
     }
@@ -689,8 +695,14 @@ extern "C" int sr_guess_extrapolate(sr_guess *g, const double *p1, const double 
 extern "C" int sr_guess_set_policy(sr_guess *g, int policy) {
     if (!g || policy < 0 || policy > 3) return set_err(SR_ERR_ARG, "policy must be 0..3");
     g->policy = policy;
-    g->net_losses = 0;
     g->net_skip = 0;
+    g->net_backoff = 8;
+    // a network that lost its last three comparisons starts the adaptive policy skipped
+    if (policy == 2 && g->net_losses >= 3) {
+        g->net_skip = g->net_backoff;
+        g->net_backoff *= 2;
+        g->net_losses = 2;
+    }
     return SR_OK;
 }
 

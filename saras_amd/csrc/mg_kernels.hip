@@ -58,6 +58,23 @@ __device__ inline void xcd_range(unsigned total, unsigned &first, unsigned &stri
     end = min(total, (x + 1u) * chunk);
 }
 
+// t = (a * n1 + b) * n2 + c for t < 2^24, split with float-reciprocal divisions (fdivu) instead of
+// integer divisions: the single-workgroup kernels are bound by their VALU issue rate
+struct Div3 {
+    unsigned n1, n2;
+    float i1, i2;
+};
+__device__ inline Div3 div3(int n1, int n2) {
+    return Div3{(unsigned)n1, (unsigned)n2, 1.0f / (float)n1, 1.0f / (float)n2};
+}
+__device__ inline void split3(const Div3 &D, int t, int &a, int &b, int &c) {
+    unsigned r = fdivu((unsigned)t, D.n2, D.i2);
+    c = t - (int)(r * D.n2);
+    unsigned q = fdivu(r, D.n1, D.i1);
+    b = (int)(r - q * D.n1);
+    a = (int)q;
+}
+
 __device__ inline bool geo_slot(const Geo &g, unsigned b, unsigned &ip, unsigned &r) {
     unsigned p = b / g.bpp;
     r = (b - p * g.bpp) * 256u + threadIdx.x;
@@ -565,8 +582,10 @@ template <typename T>
 __device__ inline void lds_fill_ghosts(const Lay &L, T *a, const Bc &bc) {
     int NX = L.nx + 2, NY = L.ny + 2, NZ = L.nz + 2;
     int n = NX * NY * NZ;
+    const Div3 dv = div3(NY, NZ);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int kp = t % NZ, r = t / NZ, jp = r % NY, ip = r / NY;
+        int ip, jp, kp;
+        split3(dv, t, ip, jp, kp);
         bool ghost = ip == 0 || ip == NX - 1 || jp == 0 || jp == NY - 1 || kp == 0 || kp == NZ - 1;
         T v;
         if (ghost && ghost_resolve(L, a, ip, jp, kp, bc, v)) a[lay_idx(L, ip, jp, kp)] = v;
@@ -577,8 +596,10 @@ __device__ inline void lds_fill_ghosts(const Lay &L, T *a, const Bc &bc) {
 template <typename T>
 __device__ inline void lds_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color) {
     int n = L.nx * L.ny * L.nz;
+    const Div3 dv = div3(L.ny, L.nz);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         if (((i + j + k) & 1) != color) continue;
         int ip = i + 1, jp = j + 1, kp = k + 1;
         const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
@@ -620,8 +641,10 @@ __global__ void __launch_bounds__(256) k_coarsest(Lay L, T *__restrict__ ug, con
         __syncthreads();
         double m = -1.0e-10;
         int nc = L.nx * L.ny * L.nz;
+        const Div3 dv = div3(L.ny, L.nz);
         for (int t = threadIdx.x; t < nc; t += blockDim.x) {
-            int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+            int i, j, k;
+            split3(dv, t, i, j, k);
             int ip = i + 1, jp = j + 1, kp = k + 1;
             int c = lay_color(ip, jp, kp);
             long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
@@ -678,8 +701,10 @@ __global__ void __launch_bounds__(256) k_project_small(Lay L, T *__restrict__ f,
                                                        double wsum) {
     int n = L.nx * L.ny * L.nz;
     double s = 0;
+    const Div3 dv = div3(L.ny, L.nz);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         s += wx[i] * wy[j] * wz[k] * (double)f[lay_idx(L, i + 1, j + 1, k + 1)];
     }
     __shared__ double red[4];
@@ -690,7 +715,8 @@ __global__ void __launch_bounds__(256) k_project_small(Lay L, T *__restrict__ f,
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) tot += red[w];
     double c = tot / wsum;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         long o = lay_idx(L, i + 1, j + 1, k + 1);
         f[o] = (T)((double)f[o] - c);
     }
@@ -858,8 +884,12 @@ template <typename T>
 __device__ inline void blk_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
                                  const Bc &bc, bool zero_start) {
     int n = L.nx * L.ny * L.kh;
+    const Div3 dv = div3(L.ny, L.kh);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int h = t % L.kh, r = t / L.kh, jp = 1 + r % L.ny, ip = 1 + r / L.ny;
+        int ip, jp, h;
+        split3(dv, t, ip, jp, h);
+        ip += 1;
+        jp += 1;
         int kb = (color + 1 + ip + jp) & 1;
         int kp = 2 * h + kb;
         if (kp < 1 || kp > L.nz) continue;
@@ -893,6 +923,18 @@ __device__ inline void blk_zero(const Lay &L, T *u) {
 template <typename T>
 __device__ inline T blk_res_at(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz,
                                const Bc &tb, int ip, int jp, int kp) {
+    if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
+        // core node: 32-bit index arithmetic, the same expression as apply_lap
+        const int sy = 2 * L.khs, sx = (L.ny + 2) * sy;
+        const int c = lay_color(ip, jp, kp);
+        const int base = (ip * (L.ny + 2) + jp) * sy;
+        const int rc = base + c * L.khs, ro = base + (c ^ 1) * L.khs;
+        const int h = kp >> 1, kb = kp & 1;
+        const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        T lu = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
+               Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - (X[2] + Y[2] + Z[2]) * u[rc + h];
+        return f[rc + h] - lu;
+    }
     int n[3] = {L.nx, L.ny, L.nz};
     int q[3] = {ip, jp, kp};
     T sign = T(1);
@@ -915,8 +957,10 @@ template <typename T>
 __device__ inline void blk_restrict(const Lay &F, const T *uf, const T *ff, const T *cx, const T *cy, const T *cz,
                                     const Bc &tb, const Lay &Cl, T *fc) {
     int n = Cl.nx * Cl.ny * Cl.nz;
+    const Div3 dv = div3(Cl.ny, Cl.nz);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % Cl.nz, r = t / Cl.nz, j = r % Cl.ny, i = r / Cl.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         int fi = 2 * i + 1, fj = 2 * j + 1, fk = 2 * k + 1;
 #define RR(a, b, e) blk_res_at<T>(F, uf, ff, cx, cy, cz, tb, fi + (a), fj + (b), fk + (e))
         T facePoints = (RR(1, 0, 0) + RR(-1, 0, 0) + RR(0, 1, 0) + RR(0, -1, 0) + RR(0, 0, 1) + RR(0, 0, -1)) * T(0.0625);
@@ -934,8 +978,10 @@ __device__ inline void blk_restrict(const Lay &F, const T *uf, const T *ff, cons
 template <typename T>
 __device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const T *e) {
     int n = F.nx * F.ny * F.nz;
+    const Div3 dv = div3(F.ny, F.nz);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % F.nz, r = t / F.nz, j = r % F.ny, i = r / F.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         int io = i & 1, jo = j & 1, ko = k & 1;
         int ci = i / 2 + 1, cj = j / 2 + 1, ck = k / 2 + 1;
 #define EE(a, b, q) e[lay_idx(Cl, ci + (a), cj + (b), ck + (q))]
@@ -969,8 +1015,10 @@ __device__ inline double blk_max_res(const Lay &L, const T *u, const T *f, const
                                      double *red) {
     double m = -1.0e-10;
     int nc = L.nx * L.ny * L.nz;
+    const Div3 dv = div3(L.ny, L.nz);
     for (int t = threadIdx.x; t < nc; t += blockDim.x) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         int ip = i + 1, jp = j + 1, kp = k + 1;
         int c = lay_color(ip, jp, kp);
         long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
@@ -991,8 +1039,10 @@ __device__ inline void blk_project(const Lay &L, T *f, const double *wx, const d
                                    double wsum, double *red) {
     int n = L.nx * L.ny * L.nz;
     double s = 0;
+    const Div3 dv = div3(L.ny, L.nz);
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         s += wx[i] * wy[j] * wz[k] * (double)f[lay_idx(L, i + 1, j + 1, k + 1)];
     }
     s = wave_sum(s);
@@ -1003,7 +1053,8 @@ __device__ inline void blk_project(const Lay &L, T *f, const double *wx, const d
     __syncthreads();
     double c = tot / wsum;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         long o = lay_idx(L, i + 1, j + 1, k + 1);
         f[o] = (T)((double)f[o] - c);
     }
@@ -1022,8 +1073,10 @@ template <typename T>
 __device__ inline void wv_fill_ghosts(const Lay &L, T *a, const Bc &bc, int lane) {
     int NX = L.nx + 2, NY = L.ny + 2, NZ = L.nz + 2;
     int n = NX * NY * NZ;
+    const Div3 dv = div3(NY, NZ);
     for (int t = lane; t < n; t += 64) {
-        int kp = t % NZ, r = t / NZ, jp = r % NY, ip = r / NY;
+        int ip, jp, kp;
+        split3(dv, t, ip, jp, kp);
         bool ghost = ip == 0 || ip == NX - 1 || jp == 0 || jp == NY - 1 || kp == 0 || kp == NZ - 1;
         T v;
         if (ghost && ghost_resolve(L, a, ip, jp, kp, bc, v)) a[lay_idx(L, ip, jp, kp)] = v;
@@ -1035,8 +1088,10 @@ template <typename T>
 __device__ inline void wv_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
                                 int lane) {
     int n = L.nx * L.ny * L.nz;
+    const Div3 dv = div3(L.ny, L.nz);
     for (int t = lane; t < n; t += 64) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         if (((i + j + k) & 1) != color) continue;
         int ip = i + 1, jp = j + 1, kp = k + 1;
         const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
@@ -1053,8 +1108,10 @@ __device__ inline double wv_max_res(const Lay &L, const T *u, const T *f, const 
                                     int lane) {
     double m = -1.0e-10;
     int nc = L.nx * L.ny * L.nz;
+    const Div3 dv = div3(L.ny, L.nz);
     for (int t = lane; t < nc; t += 64) {
-        int k = t % L.nz, r = t / L.nz, j = r % L.ny, i = r / L.ny;
+        int i, j, k;
+        split3(dv, t, i, j, k);
         int ip = i + 1, jp = j + 1, kp = k + 1;
         int c = lay_color(ip, jp, kp);
         long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);

@@ -285,8 +285,12 @@ static int solve_gcr(sr_mg *m, int &cycles, double &res, bool &conv, double &swe
     Level &V = m->lv[0];
     long sz = lay_size(V.L);
     hipStream_t s = m->stream;
-    launch_fill_ghosts<double>(V.L, V.u, pressure_bc(m, 0, 0), s);
-    HIPCHK(hipMemcpyAsync(V.t, V.u, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+    Bc bc0 = pressure_bc(m, 0, 0);
+    launch_fill_ghosts<double>(V.L, V.u, bc0, s);
+    // the second iterate buffer needs the fixed wall ghosts only; every other face ghost is
+    // rewritten as an image by each update
+    if (bc0.rule[0] == G_WALL || bc0.rule[1] == G_WALL || bc0.rule[2] == G_WALL)
+        HIPCHK(hipMemcpyAsync(V.t, V.u, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
     // no previous direction: the first step is the minimal-residual step along the cycle's result
     launch_fill<float>(m->zdir[0], sz, 0.0f, s);
     launch_outer_residual(V.L, V.u, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],

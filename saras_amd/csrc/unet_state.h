@@ -55,6 +55,7 @@ struct sr_guess {
     bool have_cache = false;
     const double *train_base = nullptr;   // base guess of the last predict (training target origin)
     int net_losses = 0, net_skip = 0;     // adaptive policy state
+    int net_backoff = 8;
     // linear predictor: a ring of the last hcap converged pairs (projected right-hand side f_i,
     // solution p_i, L p_i = f_i to the solve's tolerance) and their Gram matrix (f_i, f_j)
     int hcap = 0, hn = 0, hnext = 0;
