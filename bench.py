@@ -4,12 +4,15 @@ Grid of every Poisson workload: 257^3 nodes, periodic x/y, Neumann z, tanh z gri
 right-hand side = 8 seeded Fourier modes (projected onto the operator's range by the solver),
 V(2,2) red-black Gauss-Seidel cycles with vcDepth 7, stop on ||f - L u||_2 / ||f||_2 < 1e-10.
 
-  seq257      (default) one step = one solve of a sequence whose right-hand side evolves in time;
-              learned guess + mixed precision against plain FP64 multigrid (seq_bench)
+  rbc257      (default) one step = the pressure solve of one Rayleigh-Benard scalar_d3 time step on
+              257^3 nodes (the same grid, tanh z beta 1, PPN): learned guess + mixed precision
+              against plain FP64 multigrid warm-started as the reference does (rbc_bench)
+  rbc257x129  the same on 257x257x129 nodes (L = 2, 2, 1)
+  seq257      one step = one solve of a synthetic sequence whose right-hand side evolves in time
+              (the 8 seeded Fourier modes with time-dependent amplitudes) (seq_bench)
   poisson257  one step = one solve of the single right-hand side from a zero guess: plain FP64,
               mixed-precision refinement, GCR-accelerated mixed precision
   tgv129      one step = one Taylor-Green hydro_d3 time step at 129^3
-  rbc257x129  one step = one Rayleigh-Benard scalar_d3 time step (pressure solve timed)
 
 Inputs are resident in HBM before the timed region.  Prints one JSON line.  ``--gpus`` above 1 is
 refused: this build runs on one GPU.
@@ -28,12 +31,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (KiB per dispatch) scaled by what the same counters read for a device copy of known size
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the summaries and commands are in profiles/
-TRAFFIC_BYTES = {"k_rb_sweep<float> 257^3": 123.9e6, "k_rb_sweep<double> 257^3": 228.1e6}
+TRAFFIC_BYTES = {"k_rb_sweep<float> 257x257x257": 123.9e6, "k_rb_sweep<double> 257x257x257": 228.1e6}
 
 
-def cpu_baseline(p, g, f, max_cycles):
+def cpu_baseline(p, g, f, max_cycles, guess=None):
     """Oracle restatement of the reference algorithm (lexicographic Gauss-Seidel, one thread,
-    -Ofast build like the reference's release flags) on the same workload and stop rule."""
+    -Ofast build like the reference's release flags) on the same workload and stop rule, from a
+    zero initial guess or from `guess`."""
     import numpy as np
     from oracle import pyoracle as po
     prm = po.Params()
@@ -49,7 +53,7 @@ def cpu_baseline(p, g, f, max_cycles):
            "ztz2": g.ztz2Staggr, "ztzz": g.ztzzStaggr}
     o = po.OracleMG(prm, met, fast=True)
     t0 = time.perf_counter()
-    _, hist = o.solve(np.zeros_like(f), f)
+    _, hist = o.solve(np.zeros_like(f) if guess is None else np.array(guess, dtype=np.float64), f)
     dt = time.perf_counter() - t0
     o.close()
     return dt * 1e3, len(hist), float(hist[-1]) if len(hist) else float("nan")
@@ -123,22 +127,26 @@ def tgv_bench(args):
 
 
 def rbc_bench(args):
-    """Rayleigh-Benard convection (scalar_d3): pressure solves to 1e-10 relative residual inside the
-    time step.  The learned guess is trained online over the warm-up steps; then the same number of
-    steps is timed from the same state with (a) learned guess + mixed precision, (b) the reference's
-    warm start in FP64, (c) warm start in mixed precision."""
+    """Rayleigh-Benard convection (scalar_d3, config C5 physics): pressure solves to 1e-10 relative
+    residual inside the time step.  The learned guess (online least-squares predictor + network) is
+    trained over the warm-up steps; from that state the same number of steps is timed with
+    (a) learned guess + GCR-accelerated mixed precision, (b) the reference's warm start in plain FP64
+    (the baseline), (c) warm start in mixed refinement, (d) warm start in GCR mixed precision.
+    A step's solve time covers the guess and the solve; the right-hand side is built on the device."""
     import numpy as np
-    from saras_amd import _lib, guess, ts
+    from saras_amd import _lib, guess, mg, ts
     nz_ind = 8 if args.workload == "rbc257" else 7
     kw = dict(ind=(8, 8, nz_ind), L=(1.0, 1.0, 1.0) if nz_ind == 8 else (2.0, 2.0, 1.0), domain="PPN",
               mesh="UUD", beta=(1.0, 1.0, 1.0), scalar=True, rbc_type=2, Ra=1e7, Pr=1.0, dt=args.dt, prob_type=5,
               Re=1.0, cn_tol=1e-6, semantics=_lib.SR_SEM_PROJECTED, rel_tol=1e-10, vc_count=100, pre=2, post=2,
               res_type=2, depth=nz_ind - 1)
     n = (257, 257, 257 if nz_ind == 8 else 129)
+    log = lambda msg: print(f"[{args.workload}] {msg}", file=sys.stderr, flush=True)  # noqa: E731
     net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3, history=8)
     run = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1, **kw))
     run.attach_guess(net, train=True)
     run.initRBC(seed=0, amp=1e-3)
+    log(f"training over {args.warmup} steps")
     t0 = time.perf_counter()
     train_loss = []
     for _ in range(args.warmup):
@@ -148,21 +156,30 @@ def rbc_bench(args):
     train_s = time.perf_counter() - t0
     snap = {w: run.download(w) for w in (ts.VX, ts.VY, ts.VZ, ts.P, ts.PP, ts.T)}
     # (a) learned + mixed, frozen network (used adaptively), from the trained state
+    log("timed learned + mixed steps")
     run.attach_guess(net, train=False)
     net.set_policy(2)
     rec_l = [run.timeAdvance() for _ in range(args.steps)]
     learned = {"ms_per_solve": float(np.mean([r.solve_ms + r.guess_ms for r in rec_l])),
                "cycles_mean": float(np.mean([r.mg_cycles for r in rec_l])),
+               "cycles_per_step": [int(r.mg_cycles) for r in rec_l],
                "guess_ms_mean": float(np.mean([r.guess_ms for r in rec_l])),
                "choices": {name: int(sum(1 for r in rec_l if r.guess_choice == c))
                            for c, name in ((0, "warm"), (1, "extrapolation"), (2, "learned"), (3, "linear"))},
                "rel_guess_median": float(np.median([r.rel_guess for r in rec_l])),
-               "step_ms_mean": float(np.mean([r.step_ms for r in rec_l]))}
+               "step_ms_mean": float(np.mean([r.step_ms for r in rec_l])),
+               "sweep0_ms_mean": float(np.mean([r.sweep0_ms for r in rec_l]))}
     nu_end = run.nusselt()
+    # one real solve of this workload for the CPU baseline: the last step's right-hand side from
+    # the solution before it (the reference's warm start)
+    cpu_rhs, cpu_guess = run.download(ts.MGRHS), run.download(ts.PP2)
+    sol_l = run.download(ts.P)
     run.close()
     out = {}
-    for name, prec, direct in (("fp64_warm", _lib.SR_PREC_FP64, 0), ("mixed_warm", _lib.SR_PREC_MIXED, 0),
+    sols = {}
+    for name, prec, direct in (("fp64_warm", _lib.SR_PREC_FP64, 0), ("mixed_refinement_warm", _lib.SR_PREC_MIXED, 0),
                                ("gcr_warm", _lib.SR_PREC_MIXED_GCR, 1)):
+        log(f"timed {name} steps")
         h = ts.Hydro(ts.ts_desc(precision=prec, coarse_direct=direct, **kw))
         h.initRBC(seed=0, amp=1e-3)
         for w, a in snap.items():
@@ -171,26 +188,59 @@ def rbc_bench(args):
         out[name] = {"ms_per_solve": float(np.mean([r.solve_ms for r in rec])),
                      "cycles_mean": float(np.mean([r.mg_cycles for r in rec])),
                      "step_ms_mean": float(np.mean([r.step_ms for r in rec]))}
+        sols[name] = h.download(ts.P)
         h.close()
     net.close()
+    a_, b_ = sol_l[1:-1, 1:-1, 1:-1], sols["fp64_warm"][1:-1, 1:-1, 1:-1]
+    dP = (a_ - a_.mean()) - (b_ - b_.mean())
     base = out["fp64_warm"]["ms_per_solve"]
+    # roofline: the finest FP32 half sweep of the learned+mixed solves (HIP events in the solves)
+    p = mg.Parser.from_strings(mesh="UUD", domain="PPN", xInd=8, yInd=8, zInd=nz_ind, vcDepth=nz_ind - 1,
+                               semantics=_lib.SR_SEM_PROJECTED, precision=_lib.SR_PREC_MIXED_GCR, coarseDirect=1)
+    if nz_ind == 7:
+        p.Lx, p.Ly = 2.0, 2.0
+    solver = mg.Multigrid(mg.Grid(p), p)
+    bytes_sweep = solver.kernel_bytes(6, 0)
+    iso_ms = solver.time_kernel(6, 0, 20)
+    solver.close()
+    sweep_ms = learned["sweep0_ms_mean"]
+    achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline solve")
+        g = mg.Grid(p)
+        cpu_ms, cpu_cycles, cpu_res = cpu_baseline(p, g, cpu_rhs, args.cpu_cycles, guess=cpu_guess)
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+               "sample": f"one pressure solve of this workload (the last timed step's right-hand side, warm-started "
+                         f"from the solution before it, as the reference's mgSolve is) by the oracle's restatement "
+                         f"of the reference algorithm (lexicographic GS V(2,2), -Ofast, 1 thread): {cpu_cycles} "
+                         f"cycles to rel. residual {cpu_res:.2e}"}
     res = {
-        "metric": f"ms/pressure solve to 1e-10 rel. residual inside the RBC time step, {n[0]}x{n[1]}x{n[2]} "
-                  "(learned+mixed vs FP64 MG warm start)",
+        "metric": METRIC,
         "value": round(learned["ms_per_solve"], 4), "unit": "ms/solve", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(learned["step_ms_mean"], 4), "higher_is_better": False,
+        "warmup": args.warmup, "ms_per_step": round(learned["ms_per_solve"], 4), "higher_is_better": False,
         "scaling": "strong", "vs_baseline": None, "dtype": "f64+f32+bf16", "data": "synthetic",
         "config": {"workload": f"Rayleigh-Benard {n[0]}x{n[1]}x{n[2]} nodes, PPN, tanh z beta=1, Ra=1e7, Pr=1, "
-                               f"rbcType 2, dt={args.dt}, T perturbation 1e-3 (seed 0); pressure solve V(2,2) to "
-                               "1e-10 rel. residual; guess trained online over the warm-up steps",
-                   "learned_mixed": learned, "fp64_warm": out["fp64_warm"], "mixed_warm": out["mixed_warm"],
-                   "gcr_warm": out["gcr_warm"],
+                               f"rbcType 2, dt={args.dt}, T perturbation 1e-3 (seed 0): the pressure solve of every "
+                               "time step to 1e-10 rel. residual; learned guess trained online over the warm-up steps",
+                   "nodes": list(n), "vcDepth": nz_ind - 1,
+                   "learned_mixed": learned, "fp64_warm": out["fp64_warm"],
+                   "mixed_refinement_warm": out["mixed_refinement_warm"], "gcr_warm": out["gcr_warm"],
                    "mixed_solver": "FP32 V(2,2) cycle preconditioning flexible GCR(1) under the FP64 residual, "
                                    "exact coarsest level",
-                   "speedup_learned_mixed_vs_fp64_warm": round(base / learned["ms_per_solve"], 3),
+                   "speedup_vs_fp64_warm": round(base / learned["ms_per_solve"], 3),
+                   "pressure_max_abs_diff_vs_fp64_after_timed_steps": float(np.max(np.abs(dP))),
+                   "pressure_rel_l2_diff_vs_fp64_after_timed_steps": float(np.linalg.norm(dP) /
+                                                                          np.linalg.norm(b_ - b_.mean())),
                    "train_seconds": round(train_s, 2), "train_loss_first": train_loss[:3],
                    "train_loss_last": train_loss[-3:], "nusselt_end": nu_end},
-        "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float> finest level (FP32 V-cycle)",
+                     "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": TRAFFIC_BYTES.get(f"k_rb_sweep<float> {n[0]}x{n[1]}x{n[2]}"),
+                     "bytes_per_launch": bytes_sweep, "avg_launch_ms_in_solve": round(sweep_ms, 5),
+                     "avg_launch_ms_isolated": round(iso_ms, 5)},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(res))
 
@@ -252,7 +302,7 @@ def seq_bench(args):
     iso_ms = solver.time_kernel(6, 0, 20)
     solver.close()
     achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else None
-    traffic = TRAFFIC_BYTES.get("k_rb_sweep<float> 257^3")
+    traffic = TRAFFIC_BYTES.get("k_rb_sweep<float> 257x257x257")
     cpu = None
     if not args.no_cpu:
         pc, gc, fc = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)
@@ -294,7 +344,7 @@ def main():
                          "guess trains over them; 2 otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cycles", type=int, default=60)
-    ap.add_argument("--workload", default="seq257", choices=["seq257", "poisson257", "tgv129", "rbc257", "rbc257x129"])
+    ap.add_argument("--workload", default="rbc257", choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129"])
     ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -394,7 +444,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<double> finest level",
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": TRAFFIC_BYTES.get("k_rb_sweep<double> 257^3"),
+                     "traffic": TRAFFIC_BYTES.get("k_rb_sweep<double> 257x257x257"),
                      "bytes_per_launch": bytes_sweep, "avg_launch_ms_in_solve": round(sweep_ms, 5),
                      "avg_launch_ms_isolated": round(iso_ms, 5)},
         "cpu_baseline": cpu,

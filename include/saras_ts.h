@@ -66,6 +66,7 @@ typedef struct sr_ts_stats {
     int guess_choice;               /* sr_guess_stats.choice, -1 without a guess                   */
     double rel_guess;               /* relative residual of the guess handed to the solve          */
     double train_loss;              /* sr_guess_train's relative loss, -1 when not training        */
+    double sweep0_ms;               /* sr_mg_stats.sweep0_ms of the pressure solve                 */
 } sr_ts_stats;
 
 /* hydro_d3::hydro_d3 (src/solvers/hydro/hydro_d3.cc:63-119) without the initial condition */

@@ -673,6 +673,7 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
         for (int c = 0; c < 4; c++) st->jacobi_iters[c] = t->jac[c];
         st->solve_ms = std::chrono::duration<double, std::milli>(ts1 - ts0).count();
         st->guess_ms = guess_ms;
+        st->sweep0_ms = ms.sweep0_ms;
         st->guess_choice = choice;
         st->rel_guess = rel_guess;
         st->train_loss = loss;
