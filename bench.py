@@ -30,8 +30,8 @@ METRIC = "ms/Poisson solve to 1e-10 rel. residual, 257³ (learned+mixed vs FP64 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (KiB per dispatch) scaled by what the same counters read for a device copy of known size
-# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the summaries and commands are in profiles/
-TRAFFIC_BYTES = {"k_rb_sweep<float> 257x257x257": 123.9e6, "k_rb_sweep<double> 257x257x257": 228.1e6}
+# (FETCH_SIZE x 2.005, WRITE_SIZE x 1.0); the summaries and commands are in profiles/
+TRAFFIC_BYTES = {"k_rb_sweep<float> 257x257x257": 124.4e6, "k_rb_sweep<double> 257x257x257": 229.0e6}
 
 
 def cpu_baseline(p, g, f, max_cycles, guess=None):
@@ -423,6 +423,22 @@ def main():
              "max_abs_diff_vs_fp64": float(np.max(np.abs(dm))),
              "rel_l2_diff_vs_fp64": float(np.linalg.norm(dm) / np.linalg.norm(b - b.mean())),
              "speedup_vs_fp64": round(ms / ms_m, 3)}
+    # GCR-accelerated mixed precision with the exact coarsest level, same stop rule
+    pg = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)[0]
+    pg.precision, pg.coarseDirect = mg._L.SR_PREC_MIXED_GCR, 1
+    solver_g = mg.Multigrid(g, pg)
+    ms_g, stats_g = run(solver_g)
+    solg = d_p.download()
+    sweep32_ms = float(np.mean([s.sweep0_ms for s in stats_g]))
+    solver_g.close()
+    a = solg[1:-1, 1:-1, 1:-1]
+    dg = (a - a.mean()) - (b - b.mean())
+    gcr = {"ms_per_solve": round(ms_g, 4), "cycles": int(np.median([s.cycles for s in stats_g])),
+           "final_rel_residual": max(s.residual for s in stats_g),
+           "converged": all(s.converged for s in stats_g),
+           "max_abs_diff_vs_fp64": float(np.max(np.abs(dg))),
+           "rel_l2_diff_vs_fp64": float(np.linalg.norm(dg) / np.linalg.norm(b - b.mean())),
+           "speedup_vs_fp64": round(ms / ms_g, 3), "fp32_finest_half_sweep_ms_in_solve": round(sweep32_ms, 5)}
 
     cpu = None
     if not args.no_cpu:
@@ -440,7 +456,8 @@ def main():
                                "V(2,2) RB-GS to 1e-10 rel. residual (plain FP64 multigrid)",
                    "nodes": [257, 257, 257], "vcDepth": p.vcDepth, "variant": "fp64-mg",
                    "cycles_per_solve": int(np.median(cycles)), "final_rel_residual": res, "converged": conv,
-                   "solve_GBs_algorithmic": round(solve_gbs, 1), "mixed_precision": mixed, "plan": plan},
+                   "solve_GBs_algorithmic": round(solve_gbs, 1), "mixed_precision": mixed,
+                   "gcr_mixed_precision": gcr, "plan": plan},
         "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<double> finest level",
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
