@@ -142,7 +142,7 @@ def rbc_bench(args):
               res_type=2, depth=nz_ind - 1)
     n = (257, 257, 257 if nz_ind == 8 else 129)
     log = lambda msg: print(f"[{args.workload}] {msg}", file=sys.stderr, flush=True)  # noqa: E731
-    net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3, history=8)
+    net = guess.GuessNet(tuple((k - 1) // 2 + 1 for k in n), fine=n, seed=0, lr=1e-3, history=args.history)
     run = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1, **kw))
     run.attach_guess(net, train=True)
     run.initRBC(seed=0, amp=1e-3)
@@ -223,7 +223,7 @@ def rbc_bench(args):
         "config": {"workload": f"Rayleigh-Benard {n[0]}x{n[1]}x{n[2]} nodes, PPN, tanh z beta=1, Ra=1e7, Pr=1, "
                                f"rbcType 2, dt={args.dt}, T perturbation 1e-3 (seed 0): the pressure solve of every "
                                "time step to 1e-10 rel. residual; learned guess trained online over the warm-up steps",
-                   "nodes": list(n), "vcDepth": nz_ind - 1,
+                   "nodes": list(n), "vcDepth": nz_ind - 1, "guess_history": args.history,
                    "learned_mixed": learned, "fp64_warm": out["fp64_warm"],
                    "mixed_refinement_warm": out["mixed_refinement_warm"], "gcr_warm": out["gcr_warm"],
                    "mixed_solver": "FP32 V(2,2) cycle preconditioning flexible GCR(1) under the FP64 residual, "
@@ -346,6 +346,8 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=60)
     ap.add_argument("--workload", default="rbc257", choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129"])
     ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
+    ap.add_argument("--history", type=int, default=6,
+                    help="converged pairs kept by the learned guess's least-squares predictor (0..8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the solver on one GPU; --gpus {args.gpus} is not supported "

@@ -106,6 +106,9 @@ int sr_ts_max_div(sr_ts *ts, double *maxdiv);
  * correction.  g = NULL detaches (the reference's warm start from the previous correction).
  * No counterpart in the reference.                                                               */
 int sr_ts_attach_guess(sr_ts *ts, sr_guess *g, int train);
+/* The handle's pressure solver (owned by the time step; valid until sr_ts_destroy), e.g. for
+ * sr_mg_history of the last solve or sr_mg_info.                                                */
+int sr_ts_pressure_solver(sr_ts *ts, sr_mg **mg);
 
 /* ---- scalar_d3 (src/solvers/scalar/scalar_d3.cc) ------------------------------------------- */
 /* the constructor's state (scalar_d3.cc:85-107: P = 1, T = 0, V = 0) with the temperature

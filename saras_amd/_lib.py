@@ -145,6 +145,7 @@ SIGNATURES = [
     ("sr_guess_predict", C.c_int, [_vp, _vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(SrGuessStats)]),
     ("sr_guess_train", C.c_int, [_vp, C.c_void_p, _dp]),
+    ("sr_ts_pressure_solver", C.c_int, [_vp, C.POINTER(C.c_void_p)]),
     ("sr_guess_observe", C.c_int, [_vp, _vp, C.c_void_p, C.c_void_p]),
     ("sr_guess_kappa", C.c_int, [_vp, _dp, _ip]),
     ("sr_guess_extrapolate", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -770,3 +770,9 @@ extern "C" int sr_ts_attach_guess(sr_ts *t, sr_guess *g, int train) {
     }
     return SR_OK;
 }
+
+extern "C" int sr_ts_pressure_solver(sr_ts *t, sr_mg **mg) {
+    if (!t || !mg) return set_err(SR_ERR_ARG, "null argument");
+    *mg = t->mg;
+    return SR_OK;
+}

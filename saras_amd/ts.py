@@ -86,8 +86,11 @@ class Hydro:
         _L.check(self.L.sr_ts_init_tgv(self.h))
 
     def timeAdvance(self):
-        _L.check(self.L.sr_ts_time_advance(self.h, C.byref(self.stats)))
-        return self.stats
+        """One step; returns this step's statistics (a new record every call)."""
+        st = _L.SrTsStats()
+        _L.check(self.L.sr_ts_time_advance(self.h, C.byref(st)))
+        self.stats = st
+        return st
 
     def deriv(self, which, order, dim):
         out = np.empty(self.shape(which))
@@ -153,3 +156,11 @@ class Hydro:
         """net: saras_amd.guess.GuessNet whose solver grid is this run's node grid (or None)."""
         _L.check(self.L.sr_ts_attach_guess(self.h, net.h if net is not None else None, int(train)))
         self._guess = net
+
+    def solve_history(self, cap=256):
+        """Relative residuals after each cycle of the last pressure solve (sr_mg_history)."""
+        m = C.c_void_p()
+        _L.check(self.L.sr_ts_pressure_solver(self.h, C.byref(m)))
+        out = np.zeros(cap)
+        n = self.L.sr_mg_history(m, out.ctypes.data_as(C.POINTER(C.c_double)), cap)
+        return out[:max(n, 0)].copy()

@@ -39,14 +39,17 @@ def test_sequence_learned_guess_converges_and_trains():
     net.close()
 
 
-@pytest.mark.parametrize("history", [0, 4])
-def test_time_steps_with_guess_follow_the_plain_run(history):
+@pytest.mark.parametrize("history,gcr", [(0, False), (4, False), (4, True)])
+def test_time_steps_with_guess_follow_the_plain_run(history, gcr):
     from saras_amd import _lib, guess, ts
     kw = dict(ind=(5, 5, 4), L=(2.0, 2.0, 1.0), domain="PPN", mesh="UUD", scalar=True, rbc_type=2, Ra=1e5,
               Pr=1.0, dt=5e-3, prob_type=5, Re=1.0, semantics=_lib.SR_SEM_PROJECTED, rel_tol=1e-11, vc_count=60,
               pre=2, post=2, res_type=2)
     plain = ts.Hydro(ts.ts_desc(**kw))
-    learned = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED, **kw))
+    if gcr:   # Krylov-accelerated mixed precision with the exact coarsest level
+        learned = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1, **kw))
+    else:
+        learned = ts.Hydro(ts.ts_desc(precision=_lib.SR_PREC_MIXED, **kw))
     net = guess.GuessNet((17, 17, 9), fine=(33, 33, 17), seed=3, history=history)
     learned.attach_guess(net, train=True)
     for h in (plain, learned):
