@@ -57,6 +57,8 @@ static double rng_uniform(uint64_t &st) {
 
 static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
     g->d = *desc;
+    if (desc->history < 0 || desc->history > SR_GUESS_MAX_HISTORY)
+        return set_err(SR_ERR_ARG, "history must be 0..SR_GUESS_MAX_HISTORY");
     for (int k = 0; k < 3; k++) {
         int n = desc->n[k];
         int m = n - 1;
@@ -115,8 +117,6 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
     RETCHK(dalloc(g, &g->rpart, 4096L * 4));
     RETCHK(dalloc(g, &g->dsc, 16));
     HIPCHK(hipHostMalloc((void **)&g->hsc, sizeof(double) * 16, hipHostMallocDefault));
-    if (desc->history < 0 || desc->history > SR_GUESS_MAX_HISTORY)
-        return set_err(SR_ERR_ARG, "history must be 0..SR_GUESS_MAX_HISTORY");
     g->hcap = desc->history;
     if (g->hcap > 0) {
         for (int i = 0; i < g->hcap; i++) {

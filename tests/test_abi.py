@@ -84,6 +84,13 @@ def test_guess_create_rejects_bad_grid_without_device():
     assert b"2^k + 1" in lib.sr_last_error()
     d.n[2] = 5
     assert lib.sr_guess_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+    # the least-squares predictor keeps at most SR_GUESS_MAX_HISTORY pairs
+    d.n[2] = 17
+    for bad in (-1, 9):
+        d.history = bad
+        assert lib.sr_guess_create(C.byref(d), C.byref(h)) == _lib.SR_ERR_ARG
+        assert b"history" in lib.sr_last_error()
+    assert h.value is None
     ci, co, lv = C.c_int(), C.c_int(), C.c_int()
     assert lib.sr_guess_layer_info(4, C.byref(ci), C.byref(co), C.byref(lv)) == 0
     assert (ci.value, co.value, lv.value) == (32, 64, 2)
