@@ -97,7 +97,7 @@ static void plan_coarse(sr_mg *m, int &first, CoarseDesc &D, size_t &lds, size_t
         t.last = depth;
         if (depth - f + 1 > 8) continue;
         for (int q = 0; q <= depth - f; q++) t.L[q] = m->lv[f + q].L;
-        size_t bytes = coarse_cycle_plan<T>(t);
+        size_t bytes = coarse_cycle_plan<T>(t, limit);
         if (bytes <= limit) {
             first = f;
             D = t;

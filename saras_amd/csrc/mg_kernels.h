@@ -63,7 +63,7 @@ void launch_coarsest(const Lay &L, T *u, const T *f, const T *cx, const T *cy, c
                      double tol, int max_count, int *iters, hipStream_t s);
 
 // one-workgroup coarse end of the V-cycle: fills the LDS offsets of D and returns the LDS bytes
-template <typename T> size_t coarse_cycle_plan(CoarseDesc &D);
+template <typename T> size_t coarse_cycle_plan(CoarseDesc &D, size_t limit);
 template <typename T> int coarse_cycle_attr(size_t lds);
 template <typename T>
 void launch_coarse_cycle(const CoarseDesc &D, size_t lds, T *u_first, const T *f_first, int *iters, hipStream_t s);

@@ -918,41 +918,53 @@ __device__ inline void blk_zero(const Lay &L, T *u) {
     __syncthreads();
 }
 
-// residual of level L at padded node (ip, jp, kp) as the restriction reads it: ghost positions
-// follow the residual-pad rules (zero, or the value at the periodic / mirror source)
+// Residual of level L at padded node (ip, jp, kp) as the restriction reads it (residual-pad rules
+// tb).  The position is resolved without branches: a ghost position is mapped to the core node
+// whose residual it carries (periodic / mirror / antisymmetric image, factor +-1) or given factor 0
+// (zero pads), and the core residual f - L u (the expression of apply_lap) is evaluated there, so
+// the 27 evaluations of a restriction can overlap their loads.
 template <typename T>
-__device__ inline T blk_res_at(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz,
-                               const Bc &tb, int ip, int jp, int kp) {
-    if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
-        // core node: 32-bit index arithmetic, the same expression as apply_lap
-        const int sy = 2 * L.khs, sx = (L.ny + 2) * sy;
-        const int c = lay_color(ip, jp, kp);
-        const int base = (ip * (L.ny + 2) + jp) * sy;
-        const int rc = base + c * L.khs, ro = base + (c ^ 1) * L.khs;
-        const int h = kp >> 1, kb = kp & 1;
-        const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
-        T lu = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
-               Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - (X[2] + Y[2] + Z[2]) * u[rc + h];
-        return f[rc + h] - lu;
-    }
-    int n[3] = {L.nx, L.ny, L.nz};
+__device__ inline T blk_res_src(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz,
+                                const Bc &tb, int ip, int jp, int kp) {
+    const int n[3] = {L.nx, L.ny, L.nz};
     int q[3] = {ip, jp, kp};
-    T sign = T(1);
-    for (int d = 2; d >= 0; --d) {
-        int g = q[d];
-        if (g != 0 && g != n[d] + 1) continue;
-        int r = tb.rule[d];
-        if (r == G_ZERO || r == G_WALL || r == G_KEEP) return T(0);
-        q[d] = ghost_src(r, g, n[d]);
-        if (r == G_ANTI) sign = -sign;
+    int fac = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int g = q[d], r = tb.rule[d];
+        const bool ghost = g == 0 || g == n[d] + 1;
+        const bool image = r <= G_ANTI;
+        fac = !ghost ? fac : !image ? 0 : r == G_ANTI ? -fac : fac;
+        q[d] = !ghost ? g : image ? ghost_src(r, g, n[d]) : 1;
     }
-    int c = lay_color(q[0], q[1], q[2]);
-    long rc = lay_row(L, q[0], q[1], c), ro = lay_row(L, q[0], q[1], c ^ 1);
-    int h = q[2] >> 1, kb = q[2] & 1;
-    T lu = apply_lap<T, T>(L, u, rc, ro, h, kb, cx + 4 * q[0], cy + 4 * q[1], cz + 4 * q[2]);
-    return sign * (f[rc + h] - lu);
+    const int sy = 2 * L.khs, sx = (L.ny + 2) * sy;
+    const int c = lay_color(q[0], q[1], q[2]);
+    const int base = (q[0] * (L.ny + 2) + q[1]) * sy;
+    const int rc = base + c * L.khs, ro = base + (c ^ 1) * L.khs;
+    const int h = q[2] >> 1, kb = q[2] & 1;
+    const T *X = cx + 4 * q[0], *Y = cy + 4 * q[1], *Z = cz + 4 * q[2];
+    T lu = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
+           Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - (X[2] + Y[2] + Z[2]) * u[rc + h];
+    T res = f[rc + h] - lu;
+    return fac == 0 ? T(0) : fac < 0 ? -res : res;
 }
 
+// 27-point full weighting (poisson_d3.cc:250-298) of the residuals R(a, b, e) at the fine offsets
+// a, b, e in {-1, 0, 1} around a coarse node's fine position: faces 1/16, edges 1/32, vertices
+// 1/64, centre 1/8, summed in this order
+template <typename T, typename F>
+__device__ inline T full_weight27(F &&R) {
+    T facePoints = (R(1, 0, 0) + R(-1, 0, 0) + R(0, 1, 0) + R(0, -1, 0) + R(0, 0, 1) + R(0, 0, -1)) * T(0.0625);
+    T edgePoints = (R(1, 1, 0) + R(1, -1, 0) + R(-1, -1, 0) + R(-1, 1, 0) + R(0, 1, 1) + R(0, -1, 1) + R(0, -1, -1) +
+                    R(0, 1, -1) + R(1, 0, 1) + R(1, 0, -1) + R(-1, 0, -1) + R(-1, 0, 1)) *
+                   T(0.03125);
+    T vertPoints = (R(1, 1, 1) + R(1, 1, -1) + R(1, -1, 1) + R(-1, 1, 1) + R(1, -1, -1) + R(-1, 1, -1) + R(-1, -1, 1) +
+                    R(-1, -1, -1)) *
+                   T(0.015625);
+    return facePoints + edgePoints + vertPoints + R(0, 0, 0) * T(0.125);
+}
+
+// one thread per coarse node, its 27 fine residuals evaluated in turn
 template <typename T>
 __device__ inline void blk_restrict(const Lay &F, const T *uf, const T *ff, const T *cx, const T *cy, const T *cz,
                                     const Bc &tb, const Lay &Cl, T *fc) {
@@ -962,50 +974,100 @@ __device__ inline void blk_restrict(const Lay &F, const T *uf, const T *ff, cons
         int i, j, k;
         split3(dv, t, i, j, k);
         int fi = 2 * i + 1, fj = 2 * j + 1, fk = 2 * k + 1;
-#define RR(a, b, e) blk_res_at<T>(F, uf, ff, cx, cy, cz, tb, fi + (a), fj + (b), fk + (e))
-        T facePoints = (RR(1, 0, 0) + RR(-1, 0, 0) + RR(0, 1, 0) + RR(0, -1, 0) + RR(0, 0, 1) + RR(0, 0, -1)) * T(0.0625);
-        T edgePoints = (RR(1, 1, 0) + RR(1, -1, 0) + RR(-1, -1, 0) + RR(-1, 1, 0) + RR(0, 1, 1) + RR(0, -1, 1) +
-                        RR(0, -1, -1) + RR(0, 1, -1) + RR(1, 0, 1) + RR(1, 0, -1) + RR(-1, 0, -1) + RR(-1, 0, 1)) *
-                       T(0.03125);
-        T vertPoints = (RR(1, 1, 1) + RR(1, 1, -1) + RR(1, -1, 1) + RR(-1, 1, 1) + RR(1, -1, -1) + RR(-1, 1, -1) +
-                        RR(-1, -1, 1) + RR(-1, -1, -1)) * T(0.015625);
-        fc[lay_idx(Cl, i + 1, j + 1, k + 1)] = facePoints + edgePoints + vertPoints + RR(0, 0, 0) * T(0.125);
-#undef RR
+        fc[lay_idx(Cl, i + 1, j + 1, k + 1)] = full_weight27<T>([&](int a, int b, int e) {
+            return blk_res_src<T>(F, uf, ff, cx, cy, cz, tb, fi + a, fj + b, fk + e);
+        });
     }
     __syncthreads();
 }
 
+// The same restriction with one thread per (coarse node, tap): rounds of kTapNodes coarse nodes,
+// whose 27 x kTapNodes fine residuals are evaluated at once into scr, then weighted by one thread
+// per node with the same arithmetic as blk_restrict.  For the smallest levels, where one thread
+// per coarse node leaves most of the workgroup idle behind 27 dependent residual evaluations.
+constexpr int kTapNodes = 37;
+constexpr int kTapScratch = 27 * kTapNodes;
+
 template <typename T>
-__device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const T *e) {
-    int n = F.nx * F.ny * F.nz;
+__device__ inline void blk_restrict_taps(const Lay &F, const T *uf, const T *ff, const T *cx, const T *cy,
+                                         const T *cz, const Bc &tb, const Lay &Cl, T *fc, T *scr) {
+    const int n = Cl.nx * Cl.ny * Cl.nz;
+    const Div3 dv = div3(Cl.ny, Cl.nz);
+    const int t = threadIdx.x;
+    for (int first = 0; first < n; first += kTapNodes) {
+        if (t < kTapScratch) {
+            int node = first + t / 27, tap = t % 27;
+            if (node < n) {
+                int i, j, k;
+                split3(dv, node, i, j, k);
+                scr[t] = blk_res_src<T>(F, uf, ff, cx, cy, cz, tb, 2 * i + tap / 9, 2 * j + (tap / 3) % 3,
+                                       2 * k + tap % 3);
+            }
+        }
+        __syncthreads();
+        if (t < kTapNodes && first + t < n) {
+            const T *R = scr + 27 * t;
+            int i, j, k;
+            split3(dv, first + t, i, j, k);
+            fc[lay_idx(Cl, i + 1, j + 1, k + 1)] =
+                full_weight27<T>([&](int a, int b, int e) { return R[(a + 1) * 9 + (b + 1) * 3 + (e + 1)]; });
+        }
+        __syncthreads();
+    }
+}
+
+// uf += P e on the core nodes of the fine level (trilinear, poisson_d3.cc:301-388).  The items
+// t, t + blockDim, t + 2 blockDim of a thread update distinct nodes, so their loads are all issued
+// before the first update is stored.  With `images` every updated node also rewrites its face-ghost
+// images (all smoothing ghost rules are images), which replaces a ghost fill of the whole level.
+constexpr int kBlkBatch = 3;
+
+template <typename T>
+__device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const T *e, const Bc &bc, bool images) {
+    const int n = F.nx * F.ny * F.nz;
     const Div3 dv = div3(F.ny, F.nz);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        int i, j, k;
-        split3(dv, t, i, j, k);
-        int io = i & 1, jo = j & 1, ko = k & 1;
-        int ci = i / 2 + 1, cj = j / 2 + 1, ck = k / 2 + 1;
-#define EE(a, b, q) e[lay_idx(Cl, ci + (a), cj + (b), ck + (q))]
-        T v;
-        if (io && jo && ko)
-            v = (EE(0, 0, 0) + EE(1, 0, 0) + EE(0, 1, 0) + EE(0, 0, 1) + EE(1, 1, 0) + EE(1, 0, 1) + EE(0, 1, 1) +
-                 EE(1, 1, 1)) / T(8.0);
-        else if (io && jo)
-            v = (EE(0, 0, 0) + EE(0, 1, 0) + EE(1, 0, 0) + EE(1, 1, 0)) / T(4.0);
-        else if (io && ko)
-            v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(1, 0, 0) + EE(1, 0, 1)) / T(4.0);
-        else if (io)
-            v = (EE(0, 0, 0) + EE(1, 0, 0)) / T(2.0);
-        else if (jo && ko)
-            v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(0, 1, 0) + EE(0, 1, 1)) / T(4.0);
-        else if (jo)
-            v = (EE(0, 0, 0) + EE(0, 1, 0)) / T(2.0);
-        else if (ko)
-            v = (EE(0, 0, 0) + EE(0, 0, 1)) / T(2.0);
-        else
-            v = EE(0, 0, 0);
+    for (int t0 = threadIdx.x; t0 < n; t0 += kBlkBatch * blockDim.x) {
+        T val[kBlkBatch];
+        int pi[kBlkBatch], pj[kBlkBatch], pk[kBlkBatch];
+#pragma unroll
+        for (int b = 0; b < kBlkBatch; b++) {
+            const int t = t0 + b * (int)blockDim.x;
+            if (t >= n) continue;
+            int i, j, k;
+            split3(dv, t, i, j, k);
+            int io = i & 1, jo = j & 1, ko = k & 1;
+            int ci = i / 2 + 1, cj = j / 2 + 1, ck = k / 2 + 1;
+#define EE(a, b2, q) e[lay_idx(Cl, ci + (a), cj + (b2), ck + (q))]
+            T v;
+            if (io && jo && ko)
+                v = (EE(0, 0, 0) + EE(1, 0, 0) + EE(0, 1, 0) + EE(0, 0, 1) + EE(1, 1, 0) + EE(1, 0, 1) + EE(0, 1, 1) +
+                     EE(1, 1, 1)) / T(8.0);
+            else if (io && jo)
+                v = (EE(0, 0, 0) + EE(0, 1, 0) + EE(1, 0, 0) + EE(1, 1, 0)) / T(4.0);
+            else if (io && ko)
+                v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(1, 0, 0) + EE(1, 0, 1)) / T(4.0);
+            else if (io)
+                v = (EE(0, 0, 0) + EE(1, 0, 0)) / T(2.0);
+            else if (jo && ko)
+                v = (EE(0, 0, 0) + EE(0, 0, 1) + EE(0, 1, 0) + EE(0, 1, 1)) / T(4.0);
+            else if (jo)
+                v = (EE(0, 0, 0) + EE(0, 1, 0)) / T(2.0);
+            else if (ko)
+                v = (EE(0, 0, 0) + EE(0, 0, 1)) / T(2.0);
+            else
+                v = EE(0, 0, 0);
 #undef EE
-        long o = lay_idx(F, i + 1, j + 1, k + 1);
-        uf[o] = uf[o] + v;
+            pi[b] = i + 1;
+            pj[b] = j + 1;
+            pk[b] = k + 1;
+            val[b] = uf[lay_idx(F, i + 1, j + 1, k + 1)] + v;
+        }
+#pragma unroll
+        for (int b = 0; b < kBlkBatch; b++) {
+            if (t0 + b * (int)blockDim.x >= n) continue;
+            uf[lay_idx(F, pi[b], pj[b], pk[b])] = val[b];
+            if (images) write_images(F, uf, pi[b], pj[b], pk[b], val[b], bc);
+        }
     }
     __syncthreads();
 }
@@ -1175,7 +1237,11 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
         if (D.pre == 0) {
             blk_zero<T>(L, UU(q));
         }
-        blk_restrict<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, D.L[q + 1], FF(q + 1));
+        const Lay &C = D.L[q + 1];
+        if (D.off_s >= 0 && C.nx * C.ny * C.nz <= D.tap_max)
+            blk_restrict_taps<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, C, FF(q + 1), base + D.off_s);
+        else
+            blk_restrict<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, C, FF(q + 1));
     }
     // coarsest solve (poisson_d3.cc:190-247 with the red-black sweep)
     {
@@ -1208,10 +1274,11 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
         }
     }
     // up leg
+    const bool images = D.bc.rule[0] <= G_ANTI && D.bc.rule[1] <= G_ANTI && D.bc.rule[2] <= G_ANTI;
     for (int q = nl - 2; q >= 0; q--) {
         const Lay &L = D.L[q];
-        blk_prolong_add<T>(L, UU(q), D.L[q + 1], UU(q + 1));
-        lds_fill_ghosts(L, UU(q), D.bc);
+        blk_prolong_add<T>(L, UU(q), D.L[q + 1], UU(q + 1), D.bc, images);
+        if (!images) lds_fill_ghosts(L, UU(q), D.bc);
         for (int s = 0; s < D.post; s++) {
             blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0, D.bc, false);
             blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 1, D.bc, false);
@@ -1428,7 +1495,7 @@ void launch_coarsest_direct(const Lay &L, T *u, const T *f, const T *pt, int nc,
 }
 
 template <typename T>
-size_t coarse_cycle_plan(CoarseDesc &D) {
+size_t coarse_cycle_plan(CoarseDesc &D, size_t limit) {
     size_t off = 0;
     auto take = [&](size_t n) { size_t o = off; off += (n + 7) & ~(size_t)7; return (int)o; };
     for (int q = 0; q <= D.last - D.first; q++) {
@@ -1439,6 +1506,10 @@ size_t coarse_cycle_plan(CoarseDesc &D) {
         D.off_c[q][1] = take(4 * (size_t)(L.ny + 2));
         D.off_c[q][2] = take(4 * (size_t)(L.nz + 2));
     }
+    // scratch of the tap-parallel restriction, where it fits
+    D.off_s = -1;
+    D.tap_max = 4 * kTapNodes;
+    if ((off + kTapScratch + 8) * sizeof(T) <= limit) D.off_s = take(kTapScratch);
     return off * sizeof(T);
 }
 
@@ -1477,8 +1548,8 @@ int coarse_cycle_attr(size_t lds) {
                                           double, hipStream_t);
 SR_INST(double)
 SR_INST(float)
-template size_t coarse_cycle_plan<double>(CoarseDesc &);
-template size_t coarse_cycle_plan<float>(CoarseDesc &);
+template size_t coarse_cycle_plan<double>(CoarseDesc &, size_t);
+template size_t coarse_cycle_plan<float>(CoarseDesc &, size_t);
 template void launch_coarse_cycle<double>(const CoarseDesc &, size_t, double *, const double *, int *, hipStream_t);
 template void launch_coarse_cycle<float>(const CoarseDesc &, size_t, float *, const float *, int *, hipStream_t);
 template int coarse_cycle_attr<double>(size_t);

@@ -74,6 +74,8 @@ struct CoarseDesc {
     double wcsum;
     const void *pinv;   // dense coarsest inverse, transposed (T), or null: Gauss-Seidel to tol
     int ncd;            // coarsest core nodes (rows of pinv)
+    int off_s;          // scratch of the tap-parallel restriction, or -1 (none)
+    int tap_max;        // coarse levels of at most this many core nodes use it
 };
 
 }  // namespace sr
