@@ -62,7 +62,9 @@ typedef struct sr_ts_stats {
     int mg_cycles;                  /* V-cycles of the pressure solve                              */
     double step_ms;                 /* wall time of the call                                       */
     double solve_ms;                /* wall time of the pressure solve (guess excluded)            */
-    double guess_ms;                /* wall time of sr_guess_predict (0 without a guess)           */
+    double guess_ms;                /* wall time of the guess: sr_guess_predict before the solve,  */
+                                    /* sr_guess_train, sr_guess_observe and the copy of the solution */
+                                    /* kept as its next input after it (0 without a guess)         */
     int guess_choice;               /* sr_guess_stats.choice, -1 without a guess                   */
     double rel_guess;               /* relative residual of the guess handed to the solve          */
     double train_loss;              /* sr_guess_train's relative loss, -1 when not training        */

@@ -535,12 +535,19 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
     int base_choice = 1;
     double rel_base = 0.0;
     bool rfield_ok = false;
+    // whether the network's correction is tried on top of the base
+    bool use_net = g->policy != 3;
+    if (g->policy == 2 && g->net_skip > 0) {
+        g->net_skip -= 1;
+        use_net = false;
+    }
     g->have_fit = false;
     if (linear_on(g, m)) RETCHK(lin_fit(g, m, rhs));
     if (linear_on(g, m) && g->hn >= 1) {
-        RETCHK(lin_apply(g, g->xlin));
+        // without the network the combination is the guess itself: written straight to p_out
+        base = use_net ? g->xlin : p_out;
+        RETCHK(lin_apply(g, use_net ? g->xlin : p_out));
         rel_lin = lin_rel(g);
-        base = g->xlin;
         base_choice = 3;
         rel_base = rel_lin;
     } else {
@@ -552,11 +559,6 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
         rel_base = rel_ex;
     }
     // the network's correction on top of the base
-    bool use_net = g->policy != 3;
-    if (g->policy == 2 && g->net_skip > 0) {
-        g->net_skip -= 1;
-        use_net = false;
-    }
     int choice = base_choice;
     g->have_cache = false;
     if (use_net) {
@@ -602,7 +604,7 @@ extern "C" int sr_guess_predict(sr_guess *g, sr_mg *m, const double *p1, const d
     }
     if (choice != 2) {
         const double *src = choice == 0 ? p1 : base;
-        HIPCHK(hipMemcpyAsync(p_out, src, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
+        if (src != p_out) HIPCHK(hipMemcpyAsync(p_out, src, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
@@ -631,7 +633,9 @@ extern "C" int sr_guess_observe(sr_guess *g, sr_mg *m, const double *rhs, const 
     // the pair goes to the next ring slot (the oldest once the ring is full)
     int slot = g->hnext;
     hipStream_t s = g->s;
-    HIPCHK(hipMemcpyAsync(g->hf[slot], g->fcur, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
+    // the projected right-hand side changes places with the slot's buffer (the oldest pair's, now
+    // free), the solution is copied
+    std::swap(g->hf[slot], g->fcur);
     HIPCHK(hipMemcpyAsync(g->hp[slot], p_star, sizeof(double) * g->fine_pad, hipMemcpyDeviceToDevice, s));
     int n_new = std::min(g->hn + 1, g->hcap);
     for (int i = 0; i < n_new; i++) {

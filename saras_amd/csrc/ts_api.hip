@@ -646,13 +646,21 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     auto ts1 = std::chrono::steady_clock::now();
     t->mg_cycles = ms.cycles;
     if (t->guess) {
+        // the guess's bookkeeping after the solve (training, recording the solved pair) is part of
+        // its cost per step; both calls return with their work finished
+        auto tg0 = std::chrono::steady_clock::now();
         if (choice >= 0 && t->train) RETCHK(sr_guess_train(t->guess, t->Pp, &loss));
         RETCHK(sr_guess_observe(t->guess, t->mg, t->mgRHS, t->Pp));
+        guess_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg0).count();
     }
     ts_ghost_wave(t->fp, t->Pp, t->sync_p, t->s);
     if (t->guess) {
+        // the previous two solutions kept for the guess (its cost too)
+        auto tg1 = std::chrono::steady_clock::now();
         std::swap(t->Pp1, t->Pp2);
         HIPCHK(hipMemcpyAsync(t->Pp1, t->Pp, sizeof(double) * fsize(t->fp), hipMemcpyDeviceToDevice, t->s));
+        HIPCHK(hipStreamSynchronize(t->s));
+        guess_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg1).count();
     }
     t->nsolved += 1;
     ts_axpy_full(t->P, t->Pp, fsize(t->fp), t->s);
