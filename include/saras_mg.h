@@ -83,6 +83,9 @@ typedef struct sr_mg_stats {
     double sweep0_ms;            /* mean device time of one finest-level red-black half sweep over
                                     the solve, from HIP events recorded on the solver's stream
                                     around the finest pre-smoothing sweeps of every cycle        */
+    double residual0;            /* stop-rule residual of the initial guess, evaluated by the mixed
+                                    precisions before their first cycle; -1 for the FP64 path,
+                                    whose loop (as the reference's) does not evaluate it         */
 } sr_mg_stats;
 
 /* multigrid_d3::multigrid_d3 (lib/poisson/poisson_d3.cc:64-87): sizes, coefficients, metrics,

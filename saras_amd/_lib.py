@@ -34,7 +34,7 @@ class SrMgDesc(C.Structure):
 
 class SrMgStats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("converged", C.c_int), ("residual", C.c_double), ("solve_ms", C.c_double),
-                ("sweep0_ms", C.c_double)]
+                ("sweep0_ms", C.c_double), ("residual0", C.c_double)]
 
 
 SR_TS_VX, SR_TS_VY, SR_TS_VZ, SR_TS_P, SR_TS_PP, SR_TS_MGRHS = 0, 1, 2, 3, 4, 5

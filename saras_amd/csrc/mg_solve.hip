@@ -216,6 +216,7 @@ static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &s
     conv = false;
     cycles = 0;
     res = r;
+    m->res0 = r;
     if (m->d.semantics == SR_SEM_PROJECTED && r < m->d.relTolerance) { conv = true; return SR_OK; }
     double prev = 0.0, tsum = 0.0;
     int tcount = 0;
@@ -303,6 +304,7 @@ static int solve_gcr(sr_mg *m, int &cycles, double &res, bool &conv, double &swe
     conv = false;
     cycles = 0;
     res = r;
+    m->res0 = r;
     if (m->d.semantics == SR_SEM_PROJECTED && r < m->d.relTolerance) { conv = true; return SR_OK; }
     double prev = 0.0, tsum = 0.0;
     int tcount = 0;
@@ -373,6 +375,7 @@ extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device
     HIPCHK(hipGetLastError());
     int cycles = 0;
     double res = 0.0;
+    m->res0 = -1.0;
     bool conv = false;
     double sweep_ms = 0.0;
     Level &V = m->lv[0];
@@ -400,6 +403,7 @@ extern "C" int sr_mg_solve(sr_mg *m, double *p, const double *rhs, int on_device
         st->cycles = cycles;
         st->converged = conv ? 1 : 0;
         st->residual = res;
+        st->residual0 = m->res0;
         st->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         st->sweep0_ms = sweep_ms;
     }

@@ -68,6 +68,7 @@ struct sr_mg {
     float *pinv32 = nullptr;
     int ncd = 0;
     float *zdir[2] = {nullptr, nullptr};
+    double res0 = -1.0;        // stop-rule residual of the last solve's initial guess (-1: not evaluated)
     double *gpart = nullptr;
     double *gcoef = nullptr;
     hipGraph_t graph_gcr_g[2] = {nullptr, nullptr};

@@ -645,6 +645,9 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     RETCHK(sr_mg_solve(t->mg, t->Pp, t->mgRHS, 1, &ms));
     auto ts1 = std::chrono::steady_clock::now();
     t->mg_cycles = ms.cycles;
+    // the solver's own evaluation of the guess, where it makes one, is exact (the predictor's fit
+    // residual loses accuracy to cancellation below about 1e-7)
+    if (choice >= 0 && ms.residual0 >= 0.0) rel_guess = ms.residual0;
     if (t->guess) {
         // the guess's bookkeeping after the solve (training, recording the solved pair) is part of
         // its cost per step; both calls return with their work finished

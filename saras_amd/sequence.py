@@ -109,7 +109,8 @@ def run(seq, variant, nsteps, ntrain=0, net=None, seed=0, lr=1e-3, on_step=None,
         stats = solver.mgSolve_device(cur.ptr, f_ptr)
         t2 = time.perf_counter()
         rec.update(ms=(t2 - t0) * 1e3, guess_ms=(t1 - t0) * 1e3, solve_ms=(t2 - t1) * 1e3, cycles=stats.cycles,
-                   converged=bool(stats.converged), residual=stats.residual, sweep0_ms=stats.sweep0_ms)
+                   converged=bool(stats.converged), residual=stats.residual, sweep0_ms=stats.sweep0_ms,
+                   residual0=stats.residual0)
         hist = solver.history()
         rec["first_cycle_residual"] = float(hist[0]) if len(hist) else None
         # the guess's bookkeeping after the solve (training, recording the pair) counts as guess time
@@ -146,4 +147,6 @@ def summary(recs, first):
         for key in ("rel_extrap", "rel_learned", "rel_warm", "rel_linear"):
             vals = [r[key] for r in sel if r.get(key, -1.0) >= 0.0]
             out[key + "_median"] = float(np.median(vals)) if vals else None
+    vals = [r["residual0"] for r in sel if r.get("residual0", -1.0) >= 0.0]
+    out["guess_residual_median"] = float(np.median(vals)) if vals else None
     return out

@@ -343,3 +343,23 @@ def test_coarse_cycle_kernel_is_planned_and_matches_per_level_path():
     assert 1 <= first <= 7 and 0 < lds <= limit
     pb2 = Problem(ind=(5, 5, 5), domain="PPN", mesh="UUD")
     assert pb2.gpu().info(0) <= pb2.depth
+
+
+@pytest.mark.parametrize("gcr", [False, True], ids=["refinement", "gcr"])
+def test_mixed_solve_reports_initial_guess_residual(gcr):
+    """sr_mg_stats.residual0: the stop rule's relative residual of the guess the solve started from
+    (the mixed precisions evaluate it before their first cycle; the FP64 loop does not, -1)."""
+    from saras_amd import _lib
+    pb, g64, gm = _mixed_pair((5, 5, 5), domain="PPN", precision=_lib.SR_PREC_MIXED_GCR if gcr else None)
+    f = pb.random_rhs(5)
+    guess = 0.01 * pb.random_rhs(6)
+    o = pb.oracle()
+    o.level = 0
+    o.array(po.P, 0)[...] = guess
+    o.impose_bc()            # the solve fills the guess's ghosts from its core before using it
+    o.array(po.R, 0)[...] = o.project_rhs(f.copy())
+    want = o.compute_error(2)
+    sm = gm.mgSolve(guess.copy(), f)
+    assert sm.converged and abs(sm.residual0 - want) <= 1e-12 * want
+    s64 = g64.mgSolve(guess.copy(), f)
+    assert s64.residual0 == -1.0
