@@ -167,6 +167,7 @@ def rbc_bench(args):
                "choices": {name: int(sum(1 for r in rec_l if r.guess_choice == c))
                            for c, name in ((0, "warm"), (1, "extrapolation"), (2, "learned"), (3, "linear"))},
                "rel_guess_median": float(np.median([r.rel_guess for r in rec_l])),
+               "rel_guess_per_step": [float(f"{r.rel_guess:.3e}") for r in rec_l],
                "step_ms_mean": float(np.mean([r.step_ms for r in rec_l])),
                "sweep0_ms_mean": float(np.mean([r.sweep0_ms for r in rec_l]))}
     nu_end = run.nusselt()

@@ -46,14 +46,13 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
                            double *part2, const double *fterms, double *out, hipStream_t s);
 // Krylov-accelerated mixed precision (flexible GCR(1), FP32 V-cycle preconditioner): the step
 // coefficients from e = M r32 and the previous direction z (coef[0] alpha, coef[1] beta; part5 holds
-// 5 doubles per block of norm_res_parts), then the update x_out = x_in + alpha (e - beta z),
-// z_next = e - beta z and the FP64 residual r = f - L x_out (stored FP32; out[0..3] as launch_norm_res)
+// 5 doubles per block of norm_res_parts), then the step x_out = x_in + alpha (e - beta z),
+// z_next = e - beta z (face-ghost images by bc and ebc); the FP64 residual of x_out is then
+// launch_outer_residual's
 void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const double *cx,
                      const double *cy, const double *cz, double *part5, double *coef, hipStream_t s);
-void launch_gcr_update(const Lay &L, const double *x_in, const float *e, const float *z, const double *f,
-                       const double *coef, double *x_out, float *z_next, float *r32, const double *cx,
-                       const double *cy, const double *cz, const Bc &bc, const Bc &ebc, double *part2,
-                       const double *fterms, double *out, hipStream_t s);
+void launch_gcr_step(const Lay &L, const double *x_in, const float *e, const float *z, const double *coef,
+                     double *x_out, float *z_next, const Bc &bc, const Bc &ebc, hipStream_t s);
 // exact coarsest solve u = P f with the dense inverse (transposed, nc x nc), ghosts filled by bc
 template <typename T>
 void launch_coarsest_direct(const Lay &L, T *u, const T *f, const T *pt, int nc, const Bc &bc, int *iters,
