@@ -125,6 +125,10 @@ int sr_ts_build_trhs(sr_ts *ts);
 int sr_ts_solve_t(sr_ts *ts, int *iters);
 /* tseries Nusselt number 1 + <w T>/kappa (tseries.cc:195-246)                                    */
 int sr_ts_nusselt(sr_ts *ts, double *nu);
+/* the run's kinematic viscosity and thermal diffusivity: hydro nu = 1/Re (hydro.cc), scalar runs by
+ * rbcType (scalar.cc:69-86); tseries reports sqrt(E)/nu for scalar runs (tseries.cc:240). kappa is
+ * 0 for a hydro run.                                                                             */
+int sr_ts_coefficients(sr_ts *ts, double *nu, double *kappa);
 
 #ifdef __cplusplus
 }

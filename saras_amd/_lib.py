@@ -131,6 +131,7 @@ SIGNATURES = [
     ("sr_ts_build_trhs", C.c_int, [_vp]),
     ("sr_ts_solve_t", C.c_int, [_vp, _ip]),
     ("sr_ts_nusselt", C.c_int, [_vp, _dp]),
+    ("sr_ts_coefficients", C.c_int, [_vp, _dp, _dp]),
     ("sr_ts_attach_guess", C.c_int, [_vp, _vp, C.c_int]),
     ("sr_guess_create", C.c_int, [C.POINTER(SrGuessDesc), C.POINTER(C.c_void_p)]),
     ("sr_guess_destroy", C.c_int, [_vp]),

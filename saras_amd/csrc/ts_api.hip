@@ -703,6 +703,13 @@ static int need_scalar(const sr_ts *t) {
     return SR_OK;
 }
 
+extern "C" int sr_ts_coefficients(sr_ts *t, double *nu, double *kappa) {
+    if (!t || !nu || !kappa) return set_err(SR_ERR_ARG, "null argument");
+    *nu = t->nu;
+    *kappa = t->d.scalar ? t->kappa : 0.0;
+    return SR_OK;
+}
+
 extern "C" int sr_ts_init_rbc(sr_ts *t, unsigned long long seed, double amp) {
     RETCHK(need_scalar(t));
     const long np = fsize(t->fp);

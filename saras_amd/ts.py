@@ -22,7 +22,7 @@ def _ptr(a):
 def ts_desc(ind=(7, 7, 7), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1.0, 1.0, 1.0), Re=1600.0,
             dt=1e-3, cn_tol=1e-6, depth=None, vc_count=5, pre=4, post=4, res_type=1, mg_tol=1e-6, prob_type=2,
             scalar=False, rbc_type=2, Ra=1e7, Pr=1.0, semantics=_L.SR_SEM_REFERENCE, rel_tol=1e-10,
-            precision=_L.SR_PREC_FP64, coarse_direct=0):
+            precision=_L.SR_PREC_FP64, coarse_direct=0, gs_smooth=1):
     """The parser fields of a hydro (or, with scalar=True, scalar) run (input/parameters.yaml keys;
     defaults: the Taylor-Green case)."""
     d = _L.SrTsDesc()
@@ -37,7 +37,7 @@ def ts_desc(ind=(7, 7, 7), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1
     d.probType = prob_type
     d.Re, d.dt, d.cnTolerance = Re, dt, cn_tol
     d.vcDepth = depth if depth is not None else min(ind) - 1
-    d.vcCount, d.preSmooth, d.postSmooth, d.resType, d.gsSmooth = vc_count, pre, post, res_type, 1
+    d.vcCount, d.preSmooth, d.postSmooth, d.resType, d.gsSmooth = vc_count, pre, post, res_type, int(gs_smooth)
     d.mgTolerance = mg_tol
     return d
 
@@ -150,6 +150,12 @@ class Hydro:
         v = C.c_double()
         _L.check(self.L.sr_ts_nusselt(self.h, C.byref(v)))
         return v.value
+
+    def coefficients(self):
+        """(nu, kappa) of the run: nu = 1/Re for hydro, by rbcType for scalar runs (kappa 0 for hydro)."""
+        nu, kappa = C.c_double(), C.c_double()
+        _L.check(self.L.sr_ts_coefficients(self.h, C.byref(nu), C.byref(kappa)))
+        return nu.value, kappa.value
 
     # -- learned initial guess of the pressure solve -------------------------------------------
     def attach_guess(self, net, train=True):

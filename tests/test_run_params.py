@@ -1,0 +1,252 @@
+"""Parameter ingestion (input/parameters.yaml keys of lib/io/parser.cc:162-235, checks of
+parser::checkData, parser.cc:249-340) and the tseries line format (lib/io/tseries.cc:174-177,
+238-243).  CPU only."""
+import os
+import subprocess
+
+import pytest
+
+from saras_amd import run
+
+# the five sections with every key parser.cc reads (values: the Taylor-Green case)
+CASE = """
+"Program":
+    "Problem Type": 2
+    "Initial Condition": 1
+    "Domain Type": "PPP"
+    "RBC Type": 1
+    "Reynolds Number": 1600
+    "Rossby Number": 12
+    "Rayleigh Number": 10000
+    "Prandtl Number": 0.71
+    "Taylor Number": 0.0
+    "X Length": 6.283185307179586
+    "Y Length": 6.283185307179586
+    "Z Length": 6.283185307179586
+    "Heating Plate": false
+    "Plate Radius": 0.25
+    "Force": 0
+"Mesh":
+    "Mesh Type": "UUU"
+    "X Beta": 1.0
+    "Y Beta": 1.0
+    "Z Beta": 1.0
+    "X Index": 5
+    "Y Index": 5
+    "Z Index": 5
+"Parallel":
+    "Number of OMP threads": 1
+    "X Number of Procs": 2
+    "Y Number of Procs": 2
+"Solver":
+    "Differentiation Scheme": 1
+    "Integration Scheme": 1
+    "Solve Tolerance": 1.0e-6
+    "Restart Run": false
+    "Use CFL Condition": false
+    "Courant Number": 0.1
+    "Time-Step": 0.001
+    "Final Time": 0.005
+    "I/O Count": 1
+    "Solution Format": 1
+    "Solution Write Interval": 0.1
+    "Restart Write Interval": 0.02
+    "Record Probes": false
+    "Probe Time Interval": 0.1
+    "Probes": >
+        [5, 5, 5]
+        [1:30:3, 7, 1:30:3]
+"Multigrid":
+    "V-Cycle Depth": 3
+    "V-Cycle Count": 5
+    "Solve Tolerance": 1.0e-6
+    "Smoothing Method": 1
+    "Pre-Smoothing Count": 4
+    "Post-Smoothing Count": 4
+    "Residual Type": 1
+    "Print Residual": false
+"""
+
+
+def _with(literal=None, **changes):
+    """CASE with some values replaced: `literal` and the keywords map key texts (spaces may be
+    written as underscores; no key contains one) to new value texts."""
+    text = CASE
+    items = {k.replace("_", " "): v for k, v in dict(literal or {}, **changes).items()}
+    for key, val in items.items():
+        lines = text.splitlines()
+        hit = [i for i, l in enumerate(lines) if l.strip().startswith(f'"{key}":')]
+        assert len(hit) == 1, key
+        i = hit[0]
+        lines[i] = lines[i].split(":")[0] + ": " + val
+        text = "\n".join(lines) + "\n"
+    return text
+
+
+def test_reads_every_key_with_parser_field_names():
+    p = run.read_parameters(CASE)
+    assert (p.probType, p.icType, p.domainType, p.rbcType) == (2, 1, "PPP", 1)
+    assert (p.Re, p.Ra, p.Pr, p.Lx) == (1600.0, 10000.0, 0.71, 6.283185307179586)
+    assert (p.meshType, p.xInd, p.yInd, p.zInd, p.npX, p.npY) == ("UUU", 5, 5, 5, 2, 2)
+    assert (p.cnTolerance, p.tStp, p.tMax, p.ioCnt) == (1e-6, 0.001, 0.005, 1)
+    assert (p.vcDepth, p.vcCount, p.mgTolerance, p.gsSmooth, p.preSmooth, p.postSmooth, p.resType) == \
+        (3, 5, 1e-6, 1, 4, 4, 1)
+    assert p.useCFL is False and p.restartFlag is False and p.printResidual is False
+    assert "[5, 5, 5]" in p.probeCoords
+
+
+def test_reads_a_file(tmp_path):
+    f = tmp_path / "parameters.yaml"
+    f.write_text(CASE)
+    assert run.read_parameters(str(f)).xInd == 5
+
+
+def test_missing_key_and_wrong_type_are_errors():
+    with pytest.raises(ValueError, match="Rossby"):
+        run.read_parameters(CASE.replace('    "Rossby Number": 12\n', ""))
+    with pytest.raises(ValueError, match="integer"):
+        run.read_parameters(_with(X_Index="5.5"))
+    with pytest.raises(ValueError, match="true or false"):
+        run.read_parameters(_with(Restart_Run="1"))
+    with pytest.raises(ValueError, match="a number"):
+        run.read_parameters(_with(Reynolds_Number="fast"))
+    # numbers are read from their text, as yaml-cpp converts scalars (1.0e5 is a string to YAML 1.1)
+    assert run.read_parameters(_with(Rayleigh_Number="1.0e5")).Ra == 1e5
+
+
+@pytest.mark.parametrize("changes,msg", [
+    (dict(Domain_Type='"PP"'), "Domain type"),
+    (dict(Mesh_Type='"UUUU"'), "Mesh type"),
+    (dict(Y_Index="0"), "Y Index"),
+    (dict(Final_Time="0.0001"), "Time step is larger"),
+    (dict(Residual_Type="3"), "printing error"),
+    # 2^5 / 2 procs = 16 < 2^(4+1): the x sub-domains cannot reach depth 4
+    ({"V-Cycle Depth": "4"}, "X-direction"),
+])
+def test_check_data_rejects_what_the_reference_rejects(changes, msg):
+    with pytest.raises(ValueError, match=msg):
+        run.read_parameters(_with(changes))
+
+
+def test_check_data_depth_limits_by_direction():
+    # one process along x and y: depth 4 is reachable on 2^5 nodes in every direction
+    p = run.read_parameters(_with({"V-Cycle Depth": "4"}, X_Number_of_Procs="1", Y_Number_of_Procs="1"))
+    assert p.vcDepth == 4
+    with pytest.raises(ValueError, match="X-direction"):
+        run.read_parameters(_with({"V-Cycle Depth": "5"}, X_Number_of_Procs="1", Y_Number_of_Procs="1"))
+    # z is never decomposed: depth 4 on 2^5 z nodes passes with x and y refined
+    p = run.read_parameters(_with({"V-Cycle Depth": "4"}, X_Number_of_Procs="1", Y_Number_of_Procs="1",
+                                  X_Index="6", Y_Index="6"))
+    with pytest.raises(ValueError, match="Z-direction"):
+        run.read_parameters(_with({"V-Cycle Depth": "5"}, X_Number_of_Procs="1", Y_Number_of_Procs="1",
+                                  X_Index="7", Y_Index="7"))
+    # processor counts below 1 become 1 (a warning in the reference)
+    assert run.read_parameters(_with(X_Number_of_Procs="0")).npX == 1
+
+
+def test_descriptor_of_a_parameter_set():
+    from saras_amd import _lib
+    p = run.read_parameters(_with(Mesh_Type='"UUD"', Domain_Type='"PPN"', Z_Beta="1.3"))
+    d = run.desc_from(p)
+    assert (d.xInd, d.yInd, d.zInd, d.vcDepth, d.vcCount, d.preSmooth, d.postSmooth) == (5, 5, 5, 3, 5, 4, 4)
+    assert (d.xPer, d.yPer, d.zPer) == (1, 1, 0)
+    assert (d.xGrid, d.yGrid, d.zGrid, d.betaZ) == (1, 1, 2, 1.3)
+    assert (d.Re, d.dt, d.cnTolerance, d.mgTolerance, d.resType, d.gsSmooth) == (1600.0, 0.001, 1e-6, 1e-6, 1, 1)
+    assert (d.probType, d.scalar, d.semantics, d.precision) == (2, 0, _lib.SR_SEM_REFERENCE, _lib.SR_PREC_FP64)
+    rbc = run.read_parameters(_with(Problem_Type="5", Force="3", RBC_Type="2", Domain_Type='"PPN"'))
+    d = run.desc_from(rbc, precision=_lib.SR_PREC_MIXED_GCR)
+    assert (d.scalar, d.probType, d.rbcType, d.Ra, d.Pr, d.precision) == (1, 5, 2, 1e4, 0.71, _lib.SR_PREC_MIXED_GCR)
+
+
+@pytest.mark.parametrize("changes,what", [
+    ({"Use CFL Condition": "true"}, "CFL"),
+    (dict(Restart_Run="true"), "Restart"),
+    (dict(Mesh_Type='"USU"'), "single-sided"),
+    (dict(Problem_Type="3"), "Problem Type 3"),
+    (dict(Force="2"), "Force 2"),
+    (dict(Problem_Type="5", Force="0"), "Force 0 for a scalar run"),
+    (dict(Initial_Condition="2"), "Initial Condition 2"),
+    (dict(Integration_Scheme="2"), "Integration Scheme"),
+])
+def test_options_not_built_are_refused(changes, what):
+    p = run.read_parameters(_with(changes))
+    with pytest.raises(NotImplementedError, match=what):
+        run.desc_from(p)
+
+
+CPP = r"""
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+int main() {
+    double v[][4] = {{0.0, 0.125, 3.5e-9, 0.001}, {0.0123456, 12345.678901234, 7.25e-3, 1e-4},
+                     {1.5, 0.333333333333, 1.0 / 3.0, 0.01}};
+    for (auto &r : v) {
+        std::cout << std::fixed << std::setw(6) << std::setprecision(4) << r[0] << "\t" << std::setw(16)
+                  << std::setprecision(8) << r[1] << "\t" << r[2] << "\t" << r[3] << std::endl;
+        std::cout << std::fixed << std::setw(6) << std::setprecision(4) << r[0] << "\t" << std::setw(16)
+                  << std::setprecision(8) << r[1] << "\t" << r[2] << std::endl;
+    }
+    std::cout << std::fixed << std::setw(6) << "Time" << "\t" << std::setw(16) << "Total energy" << "\t"
+              << "Divergence" << std::endl;
+    return 0;
+}
+"""
+
+
+def test_line_format_matches_iostream(tmp_path):
+    """The tseries lines (std::fixed, setw(6)/setprecision(4) time, setw(16)/setprecision(8) values)
+    against the same iostream statements compiled here."""
+    src = tmp_path / "fmt.cc"
+    src.write_text(CPP)
+    exe = tmp_path / "fmt"
+    subprocess.run(["g++", "-O0", "-o", str(exe), str(src)], check=True)
+    lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    rows = [(0.0, 0.125, 3.5e-9, 0.001), (0.0123456, 12345.678901234, 7.25e-3, 1e-4),
+            (1.5, 0.333333333333, 1.0 / 3.0, 0.01)]
+    ours = []
+    for t, a, b, dt in rows:
+        ours.append(run.format_line([a, b], t, dt))
+        ours.append(run.format_line([a, b], t))
+    assert lines[:-1] == ours
+    assert lines[-1] == f"{'Time':>6}\t{'Total energy':>16}\tDivergence"
+
+
+def test_time_series_writer_header_and_rows(tmp_path):
+    """The writer on a stand-in run object (fixed diagnostics): header by problem type, one row per
+    write, sqrt(E)/nu for scalar runs, abort above the divergence limit."""
+    class Fake:
+        def __init__(self, div=1e-9):
+            self.div = div
+
+        def coefficients(self):
+            return 0.01, 0.002
+
+        def energy(self):
+            return 0.25
+
+        def maxDivergence(self):
+            return self.div
+
+        def nusselt(self):
+            return 1.75
+
+    p = run.read_parameters(CASE)
+    path = tmp_path / "output" / "TimeSeries.dat"
+    w = run.TimeSeries(Fake(), p, str(path), log=lambda s: None)
+    w.write(0.0, 0.001)
+    w.write(0.001, 0.001)
+    w.close()
+    lines = path.read_text().splitlines()
+    assert lines[0] == "#VARIABLES = Time, Energy, Divergence, dt"
+    assert lines[1] == run.format_line([0.25, 1e-9], 0.0, 0.001) and len(lines) == 3
+    ps = run.read_parameters(_with(Problem_Type="5", Force="3"))
+    w = run.TimeSeries(Fake(), ps, str(path), log=lambda s: None)
+    assert w.write(0.0, 0.001) == [0.5 / 0.01, 1.75, 1e-9]
+    w.close()
+    assert path.read_text().splitlines()[0] == "#VARIABLES = Time, Energy, NusseltNo, Divergence, dt"
+    w = run.TimeSeries(Fake(div=2e5), p, str(path), log=lambda s: None)
+    with pytest.raises(RuntimeError, match="Divergence"):
+        w.write(0.0, 0.001)
+    w.close()
+    assert os.path.exists(path)
