@@ -13,6 +13,8 @@ V(2,2) red-black Gauss-Seidel cycles with vcDepth 7, stop on ||f - L u||_2 / ||f
   poisson257  one step = one solve of the single right-hand side from a zero guess: plain FP64,
               mixed-precision refinement, GCR-accelerated mixed precision
   tgv129      one step = one Taylor-Green hydro_d3 time step at 129^3
+  mixed257x16 the 257^3 case for seeds 0..15: FP64, mixed refinement and GCR mixed solves, the
+              mixed solutions' errors against the FP64 solve, seed 0 against the oracle
 
 Inputs are resident in HBM before the timed region.  Prints one JSON line.  ``--gpus`` above 1 is
 refused: this build runs on one GPU.
@@ -57,6 +59,100 @@ def cpu_baseline(p, g, f, max_cycles, guess=None):
     dt = time.perf_counter() - t0
     o.close()
     return dt * 1e3, len(hist), float(hist[-1]) if len(hist) else float("nan")
+
+
+def mixed_seeds_bench(args):
+    """The mixed-precision variant of the 257^3 stretched-grid case over 16 seeded right-hand sides:
+    each seed solved to 1e-10 by plain FP64 multigrid, by FP32 V(2,2) cycles under FP64 iterative
+    refinement and by the GCR-accelerated form, from a zero guess; the solutions' max-abs and L2
+    differences against the FP64 solve (centred: the PPN problem fixes the solution up to a
+    constant).  Seed 0 is also solved by the oracle's FP64 multigrid (red-black order, the device's
+    algorithm) on the CPU, which pins the FP64 reference itself."""
+    import numpy as np
+    from saras_amd import mg, workloads
+    from saras_amd._lib import DeviceArray
+    p, g, _ = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)
+    shape = g.shape()
+    d_rhs, d_p = DeviceArray(shape), DeviceArray(shape)
+    solvers = {}
+    for name, prec, direct in (("fp64", mg._L.SR_PREC_FP64, 0), ("mixed", mg._L.SR_PREC_MIXED, 0),
+                               ("gcr", mg._L.SR_PREC_MIXED_GCR, 1)):
+        q = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)[0]
+        q.precision, q.coarseDirect = prec, direct
+        solvers[name] = mg.Multigrid(g, q)
+    centred = lambda a: a[1:-1, 1:-1, 1:-1] - a[1:-1, 1:-1, 1:-1].mean()  # noqa: E731
+    rows, sols0, f0 = [], {}, None
+    for seed in range(16):
+        f = workloads.fourier_rhs(g.xStaggr, g.yStaggr, g.zStaggr, seed)
+        d_rhs.upload(f)
+        rec, sols = {"seed": seed}, {}
+        for name, s in solvers.items():
+            d_p.upload(np.zeros(shape))
+            t0 = time.perf_counter()
+            st = s.mgSolve_device(d_p.ptr, d_rhs.ptr)
+            rec[name + "_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+            rec[name + "_cycles"] = int(st.cycles)
+            rec[name + "_residual"] = float(st.residual)
+            sols[name] = centred(d_p.download())
+        b = sols["fp64"]
+        for name in ("mixed", "gcr"):
+            d = sols[name] - b
+            rec[name + "_max_abs_err"] = float(np.max(np.abs(d)))
+            rec[name + "_rel_l2_err"] = float(np.linalg.norm(d) / np.linalg.norm(b))
+        rows.append(rec)
+        if seed == 0:
+            sols0, f0 = sols, f
+        print(f"[mixed257x16] seed {seed}: cycles {rec['fp64_cycles']}/{rec['mixed_cycles']}/{rec['gcr_cycles']}, "
+              f"max |mixed - fp64| {rec['mixed_max_abs_err']:.2e}, max |gcr - fp64| {rec['gcr_max_abs_err']:.2e}",
+              file=sys.stderr, flush=True)
+    for s in solvers.values():
+        s.close()
+    d_rhs.free()
+    d_p.free()
+    oracle = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        prm = po.Params()
+        prm.xInd, prm.yInd, prm.zInd = p.xInd, p.yInd, p.zInd
+        prm.vcDepth, prm.vcCount = p.vcDepth, 60
+        prm.preSmooth, prm.postSmooth, prm.resType = p.preSmooth, p.postSmooth, 2
+        prm.smoother = po.SMOOTH_RB
+        prm.xPer, prm.yPer, prm.zPer = int(p.xPer), int(p.yPer), int(p.zPer)
+        prm.testPoisson, prm.semantics = 0, po.SEM_PROJECTED
+        prm.mgTolerance, prm.relTolerance = p.mgTolerance, p.relTolerance
+        prm.dXi, prm.dEt, prm.dZt = g.dXi, g.dEt, g.dZt
+        met = {"xix2": g.xix2Staggr, "xixx": g.xixxStaggr, "ety2": g.ety2Staggr, "etyy": g.etyyStaggr,
+               "ztz2": g.ztz2Staggr, "ztzz": g.ztzzStaggr}
+        o = po.OracleMG(prm, met, fast=True)
+        t0 = time.perf_counter()
+        po_sol, hist = o.solve(np.zeros(shape), f0)
+        cpu_s = time.perf_counter() - t0
+        o.close()
+        ref = centred(np.asarray(po_sol))
+        scale = float(np.max(np.abs(ref)))
+        oracle = {"seed": 0, "cycles": len(hist), "final_rel_residual": float(hist[-1]),
+                  **{f"{n}_max_abs_diff_vs_oracle": float(np.max(np.abs(sols0[n] - ref))) for n in sols0},
+                  "solution_max_abs": scale}
+        cpu = {"value": round(cpu_s * 1e3, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+               "sample": "the seed-0 solve by the oracle's FP64 multigrid (red-black V(2,2), -Ofast, 1 thread)"}
+    keys = ("fp64_ms", "mixed_ms", "gcr_ms", "fp64_cycles", "mixed_cycles", "gcr_cycles")
+    summ = {k + "_mean": float(np.mean([r[k] for r in rows])) for k in keys}
+    for n in ("mixed", "gcr"):
+        summ[n + "_max_abs_err_max"] = max(r[n + "_max_abs_err"] for r in rows)
+        summ[n + "_rel_l2_err_max"] = max(r[n + "_rel_l2_err"] for r in rows)
+    out = {
+        "metric": "ms/Poisson solve to 1e-10 rel. residual, 257^3, mixed precision over 16 seeded right-hand sides",
+        "value": round(summ["mixed_ms_mean"], 4), "unit": "ms/solve", "n_gpus": 1, "steps": 16, "warmup": 0,
+        "ms_per_step": round(summ["mixed_ms_mean"], 4), "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "stretched-grid Poisson 257^3 (PPN, tanh z beta=1), seeds 0..15 of the 8-mode "
+                               "Fourier right-hand side, each solved from zero to 1e-10 rel. residual by FP64 MG, "
+                               "FP32-in-FP64 iterative refinement and GCR-accelerated mixed precision",
+                   "summary": summ, "per_seed": rows, "oracle_seed0": oracle},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
 
 
 def tgv_bench(args):
@@ -345,7 +441,8 @@ def main():
                          "guess trains over them; 2 otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cycles", type=int, default=60)
-    ap.add_argument("--workload", default="rbc257", choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129"])
+    ap.add_argument("--workload", default="rbc257",
+                    choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129", "mixed257x16"])
     ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
     ap.add_argument("--history", type=int, default=6,
                     help="converged pairs kept by the learned guess's least-squares predictor (0..8)")
@@ -358,6 +455,9 @@ def main():
         args.warmup = 30 if args.workload in ("seq257", "rbc257", "rbc257x129") else 2
     if args.workload == "seq257":
         seq_bench(args)
+        return
+    if args.workload == "mixed257x16":
+        mixed_seeds_bench(args)
         return
     if args.workload == "tgv129":
         tgv_bench(args)
