@@ -251,7 +251,10 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
                 double m2 = met[2 * a][i * (1 << l) + 1], m1 = met[2 * a + 1][i * (1 << l) + 1];
                 c[4 * (i + 1) + 0] = m2 / (h * h) + m1 / (2.0 * h);
                 c[4 * (i + 1) + 1] = m2 / (h * h) - m1 / (2.0 * h);
-                c[4 * (i + 1) + 2] = 2.0 * m2 / (h * h);
+                // the centre coefficient 2 m2 / h^2 as the sum of the two neighbour coefficients (equal
+                // to within one rounding); the FP64 kernels form the diagonal from those two instead of
+                // loading a third value
+                c[4 * (i + 1) + 2] = c[4 * (i + 1) + 0] + c[4 * (i + 1) + 1];
             }
             V.coef_h[a] = c;
             RETCHK(upload_vec(&V.c64[a], c));

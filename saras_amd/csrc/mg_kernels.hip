@@ -157,6 +157,16 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(Lay L, T *__restrict__ a, B
 // Face-ghost images of an updated node: a node whose padded index is the source of a ghost of a
 // periodic / mirror / anti direction writes that ghost (same colour, so no other thread of this
 // half sweep reads it).  Edge and corner ghosts are not read by the 7-point stencil.
+// Diagonal of the 7-point operator at a node from the three directions' coefficient quadruples
+// (c+, c-, c0, pad).  The FP64 quadruples store c0 = c+ + c- (rounded once, at set-up), so the sum
+// is formed from c+ and c-, which the stencil loads anyway, with the same value as from c0; the FP32
+// quadruples hold each value rounded separately and use c0.
+template <typename T>
+__device__ __forceinline__ T diag3(const T *X, const T *Y, const T *Z) {
+    if constexpr (sizeof(T) == 8) return (X[0] + X[1]) + (Y[0] + Y[1]) + (Z[0] + Z[1]);
+    else return X[2] + Y[2] + Z[2];
+}
+
 template <typename T>
 __device__ inline void write_images(const Lay &L, T *u, int ip, int jp, int kp, T v, const Bc &bc) {
     int n[3] = {L.nx, L.ny, L.nz};
@@ -199,7 +209,7 @@ __global__ void __launch_bounds__(256) k_rb_sweep(Lay L, Geo G, T *__restrict__ 
     long rc = lay_row(L, ip, jp, color);
     T num;
     const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
-    T diag = X[2] + Y[2] + Z[2];
+    T diag = diag3<T>(X, Y, Z);
     if (ZERO_START) {
         num = -f[rc + h];
     } else {
@@ -228,7 +238,7 @@ __device__ inline T apply_lap(const Lay &L, const T *__restrict__ u, long row_c,
     T xm = u[row_o - sx + h], xp = u[row_o + sx + h];
     T ym = u[row_o - sy + h], yp = u[row_o + sy + h];
     T zm = u[row_o + h - 1 + kb], zp = u[row_o + h + kb];
-    return X[0] * xp + X[1] * xm + Y[0] * yp + Y[1] * ym + Z[0] * zp + Z[1] * zm - (X[2] + Y[2] + Z[2]) * c;
+    return X[0] * xp + X[1] * xm + Y[0] * yp + Y[1] * ym + Z[0] * zp + Z[1] * zm - diag3<T>(X, Y, Z) * c;
 }
 
 template <typename T, typename TO>
@@ -606,7 +616,7 @@ __device__ inline void lds_sweep(const Lay &L, T *u, const T *f, const T *cx, co
         T num = X[0] * u[lay_idx(L, ip + 1, jp, kp)] + X[1] * u[lay_idx(L, ip - 1, jp, kp)] +
                 Y[0] * u[lay_idx(L, ip, jp + 1, kp)] + Y[1] * u[lay_idx(L, ip, jp - 1, kp)] +
                 Z[0] * u[lay_idx(L, ip, jp, kp + 1)] + Z[1] * u[lay_idx(L, ip, jp, kp - 1)] - f[lay_idx(L, ip, jp, kp)];
-        u[lay_idx(L, ip, jp, kp)] = num / (X[2] + Y[2] + Z[2]);
+        u[lay_idx(L, ip, jp, kp)] = num / diag3<T>(X, Y, Z);
     }
 }
 
@@ -689,7 +699,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Lay L, Geo G, const T *__restric
     const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
     T num = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
             Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - f[rc + h];
-    tout[rc + h] = num / (X[2] + Y[2] + Z[2]);
+    tout[rc + h] = num / diag3<T>(X, Y, Z);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -744,7 +754,7 @@ __global__ void __launch_bounds__(256) k_outer_residual(Lay L, Geo G, const doub
         double uc = U(rc + h);
         const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
         double lu = X[0] * U(ro + sx + h) + X[1] * U(ro - sx + h) + Y[0] * U(ro + sy + h) + Y[1] * U(ro - sy + h) +
-                    Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) - (X[2] + Y[2] + Z[2]) * uc;
+                    Z[0] * U(ro + h + kb) + Z[1] * U(ro + h - 1 + kb) - diag3<double>(X, Y, Z) * uc;
         double rv = f[rc + h] - lu;
         r32[rc + h] = (float)rv;
         if (CORR) {
@@ -773,7 +783,7 @@ __device__ inline double lap_f(const Lay &L, const TA *__restrict__ a, long rc, 
     const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
     return X[0] * (double)a[ro + sx + h] + X[1] * (double)a[ro - sx + h] + Y[0] * (double)a[ro + sy + h] +
            Y[1] * (double)a[ro - sy + h] + Z[0] * (double)a[ro + h + kb] + Z[1] * (double)a[ro + h - 1 + kb] -
-           (X[2] + Y[2] + Z[2]) * (double)a[rc + h];
+           diag3<double>(X, Y, Z) * (double)a[rc + h];
 }
 
 // per-block sums (r q, q q, q p, r p, p p), written at the hardware block index
@@ -892,7 +902,7 @@ __device__ inline void blk_sweep(const Lay &L, T *u, const T *f, const T *cx, co
             num = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
                   Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - f[rc + h];
         }
-        T v = num / (X[2] + Y[2] + Z[2]);
+        T v = num / diag3<T>(X, Y, Z);
         u[rc + h] = v;
         write_images(L, u, ip, jp, kp, v, bc);
     }
@@ -932,7 +942,7 @@ __device__ inline T blk_res_src(const Lay &L, const T *u, const T *f, const T *c
     const int h = q[2] >> 1, kb = q[2] & 1;
     const T *X = cx + 4 * q[0], *Y = cy + 4 * q[1], *Z = cz + 4 * q[2];
     T lu = X[0] * u[ro + sx + h] + X[1] * u[ro - sx + h] + Y[0] * u[ro + sy + h] + Y[1] * u[ro - sy + h] +
-           Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - (X[2] + Y[2] + Z[2]) * u[rc + h];
+           Z[0] * u[ro + h + kb] + Z[1] * u[ro + h - 1 + kb] - diag3<T>(X, Y, Z) * u[rc + h];
     T res = f[rc + h] - lu;
     return fac == 0 ? T(0) : fac < 0 ? -res : res;
 }
@@ -1148,7 +1158,7 @@ __device__ inline void wv_sweep(const Lay &L, T *u, const T *f, const T *cx, con
         T num = X[0] * u[lay_idx(L, ip + 1, jp, kp)] + X[1] * u[lay_idx(L, ip - 1, jp, kp)] +
                 Y[0] * u[lay_idx(L, ip, jp + 1, kp)] + Y[1] * u[lay_idx(L, ip, jp - 1, kp)] +
                 Z[0] * u[lay_idx(L, ip, jp, kp + 1)] + Z[1] * u[lay_idx(L, ip, jp, kp - 1)] - f[lay_idx(L, ip, jp, kp)];
-        u[lay_idx(L, ip, jp, kp)] = num / (X[2] + Y[2] + Z[2]);
+        u[lay_idx(L, ip, jp, kp)] = num / diag3<T>(X, Y, Z);
     }
     wave_sync();
 }
