@@ -73,6 +73,9 @@ typedef struct sr_mg_desc {
     /* testPoisson only: Dirichlet wall planes, host pointers or NULL, padded 2D over the other two
        directions in (a,b) order: x lo, x hi, y lo, y hi, z lo, z hi (poisson_d3.cc:513-572)      */
     const double *wall[6];
+    /* smoothing counts of the FP32 cycle of the mixed precisions (the preconditioner: its counts need
+       not be the reference's); 0 or less: preSmooth / postSmooth                                 */
+    int innerPreSmooth, innerPostSmooth;
 } sr_mg_desc;
 
 typedef struct sr_mg_stats {

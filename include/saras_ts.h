@@ -55,6 +55,7 @@ typedef struct sr_ts_desc {
     double relTolerance;
     int precision;                  /* pressure solve: SR_PREC_FP64, _MIXED or _MIXED_GCR          */
     int coarseDirect;               /* pressure solve: sr_mg_desc.coarseDirect                     */
+    int innerPreSmooth, innerPostSmooth; /* pressure solve: sr_mg_desc.innerPreSmooth / _PostSmooth */
 } sr_ts_desc;
 
 typedef struct sr_ts_stats {

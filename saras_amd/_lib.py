@@ -29,6 +29,7 @@ class SrMgDesc(C.Structure):
         ("dXi", C.c_double), ("dEt", C.c_double), ("dZt", C.c_double),
         ("xix2", _dp), ("xixx", _dp), ("ety2", _dp), ("etyy", _dp), ("ztz2", _dp), ("ztzz", _dp),
         ("wall", _dp * 6),
+        ("innerPreSmooth", C.c_int), ("innerPostSmooth", C.c_int),
     ]
 
 
@@ -57,6 +58,7 @@ class SrTsDesc(C.Structure):
         ("scalar", C.c_int), ("rbcType", C.c_int), ("Ra", C.c_double), ("Pr", C.c_double),
         ("semantics", C.c_int), ("relTolerance", C.c_double), ("precision", C.c_int),
         ("coarseDirect", C.c_int),
+        ("innerPreSmooth", C.c_int), ("innerPostSmooth", C.c_int),
     ]
 
 

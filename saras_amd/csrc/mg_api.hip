@@ -195,8 +195,8 @@ static void finish_coarse(sr_mg *m, CoarseDesc &D, bool fp32) {
     for (int q = 0; q <= D.last - D.first; q++)
         for (int a = 0; a < 3; a++)
             D.cg[q][a] = fp32 ? (const void *)m->lv[D.first + q].c32[a] : (const void *)m->lv[D.first + q].c64[a];
-    D.pre = m->d.preSmooth;
-    D.post = m->d.postSmooth;
+    D.pre = fp32 ? m->pre32 : m->d.preSmooth;
+    D.post = fp32 ? m->post32 : m->d.postSmooth;
     D.bc = pressure_bc(m, D.first, 1);
     for (int q = 0; q < 6; q++) D.bc.wall[q] = nullptr;
     D.tbc = tmp_bc(m);
@@ -224,6 +224,8 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
     if (depth < 1 || depth >= *std::min_element(ind, ind + 3))
         return set_err(SR_ERR_ARG, "vcDepth must satisfy 1 <= vcDepth < min(xInd, yInd, zInd)");
     if (d->preSmooth < 0 || d->postSmooth < 0 || d->vcCount < 0) return set_err(SR_ERR_ARG, "negative counts");
+    m->pre32 = d->innerPreSmooth > 0 ? d->innerPreSmooth : d->preSmooth;
+    m->post32 = d->innerPostSmooth > 0 ? d->innerPostSmooth : d->postSmooth;
     const double *met[6] = {d->xix2, d->xixx, d->ety2, d->etyy, d->ztz2, d->ztzz};
     for (int q = 0; q < 6; q++)
         if (!met[q]) return set_err(SR_ERR_ARG, "metric array missing");
@@ -585,7 +587,7 @@ void sr::vcycle_fp32(sr_mg *m) {
     hipStream_t s = m->stream;
     int Lc = m->coarse_enable ? m->coarse_first32 : (1 << 20);
     int bottom = Lc <= D ? Lc : D;
-    smooth32(m, 0, d.preSmooth, 0, true, m->sw0, m->sw1);
+    smooth32(m, 0, m->pre32, 0, true, m->sw0, m->sw1);
     for (int l = 0; l < bottom; l++) {
         Level &V = m->lv[l];
         launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s);
@@ -605,12 +607,12 @@ void sr::vcycle_fp32(sr_mg *m) {
             launch_coarsest<float>(Cv.L, Cv.u32, Cv.f32, Cv.c32[0], Cv.c32[1], Cv.c32[2], error_bc(m, D, 1),
                                    d.mgTolerance, m->maxCount, m->diters, s);
         } else {
-            smooth32(m, l + 1, d.preSmooth, 1, true);
+            smooth32(m, l + 1, m->pre32, 1, true);
         }
     }
     for (int l = bottom - 1; l >= 0; l--) {
         launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s);
-        smooth32(m, l, d.postSmooth, l == 0 ? 0 : 1, false);
+        smooth32(m, l, m->post32, l == 0 ? 0 : 1, false);
     }
 }
 

@@ -225,9 +225,9 @@ static int solve_mixed(sr_mg *m, int &cycles, double &res, bool &conv, double &s
         RETCHK(run_graph2(m, cur));
         HIPCHK(hipStreamSynchronize(m->stream));
         float ms = 0.f;
-        if (m->d.preSmooth > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
+        if (m->pre32 > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
             tsum += ms;
-            tcount += 2 * m->d.preSmooth;
+            tcount += 2 * m->pre32;
         }
         cur ^= 1;
         dsol = cur ? V.t : V.u;
@@ -314,9 +314,9 @@ static int solve_gcr(sr_mg *m, int &cycles, double &res, bool &conv, double &swe
         RETCHK(run_gcr_graph(m, cur));
         HIPCHK(hipStreamSynchronize(s));
         float ms = 0.f;
-        if (m->d.preSmooth > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
+        if (m->pre32 > 0 && hipEventElapsedTime(&ms, m->sw0, m->sw1) == hipSuccess) {
             tsum += ms;
-            tcount += 2 * m->d.preSmooth;
+            tcount += 2 * m->pre32;
         }
         cur ^= 1;
         dsol = cur ? V.t : V.u;

@@ -68,6 +68,7 @@ struct sr_mg {
     float *pinv32 = nullptr;
     int ncd = 0;
     float *zdir[2] = {nullptr, nullptr};
+    int pre32 = 0, post32 = 0;  // smoothing counts of the FP32 cycle (innerPre/PostSmooth or the outer ones)
     double res0 = -1.0;        // stop-rule residual of the last solve's initial guess (-1: not evaluated)
     double *gpart = nullptr;
     double *gcoef = nullptr;

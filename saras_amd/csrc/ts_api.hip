@@ -294,6 +294,8 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
     md.semantics = d.semantics;
     md.precision = d.precision;
     md.coarseDirect = d.coarseDirect;
+    md.innerPreSmooth = d.innerPreSmooth;
+    md.innerPostSmooth = d.innerPostSmooth;
     md.mgTolerance = d.mgTolerance;
     md.relTolerance = d.relTolerance > 0.0 ? d.relTolerance : 1e-10;
     md.dXi = t->h[0]; md.dEt = t->h[1]; md.dZt = t->h[2];

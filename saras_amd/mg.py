@@ -48,6 +48,8 @@ class Parser:
     padMode: int = 0
     coarseProject: int = 0
     coarseDirect: int = 0
+    innerPreSmooth: int = 0     # FP32 cycle of the mixed precisions (0: preSmooth / postSmooth)
+    innerPostSmooth: int = 0
     relTolerance: float = 1.0e-10
 
     @staticmethod
@@ -112,6 +114,7 @@ class Multigrid:
         d.xPer, d.yPer, d.zPer = int(p.xPer), int(p.yPer), int(p.zPer)
         d.testPoisson, d.semantics, d.precision = int(p.testPoisson), p.semantics, p.precision
         d.padMode, d.coarseProject, d.coarseDirect = p.padMode, p.coarseProject, p.coarseDirect
+        d.innerPreSmooth, d.innerPostSmooth = p.innerPreSmooth, p.innerPostSmooth
         d.mgTolerance, d.relTolerance = p.mgTolerance, p.relTolerance
         d.dXi, d.dEt, d.dZt = grid.dXi, grid.dEt, grid.dZt
         for name, arr in (("xix2", grid.xix2Staggr), ("xixx", grid.xixxStaggr), ("ety2", grid.ety2Staggr),

@@ -50,7 +50,7 @@ class Problem:
         return po.OracleMG(p, metrics_from_axes(ax), walls=self.walls)
 
     # ---- HIP path ----
-    def gpu(self, precision=0, coarse_direct=0):
+    def gpu(self, precision=0, coarse_direct=0, inner=(0, 0)):
         from saras_amd import mg
         pr = mg.Parser(xInd=self.ind[0], yInd=self.ind[1], zInd=self.ind[2], Lx=self.L[0], Ly=self.L[1],
                        Lz=self.L[2], xGrid=self.gtype[0], yGrid=self.gtype[1], zGrid=self.gtype[2],
@@ -59,7 +59,8 @@ class Problem:
                        preSmooth=self.pre, postSmooth=self.post, gsSmooth=self.gs, resType=self.res_type,
                        mgTolerance=self.mg_tol, testPoisson=self.test_poisson, semantics=self.semantics,
                        padMode=self.pad_mode, coarseProject=self.coarse_project, relTolerance=self.rel_tol,
-                       precision=precision, coarseDirect=coarse_direct)
+                       precision=precision, coarseDirect=coarse_direct, innerPreSmooth=inner[0],
+                       innerPostSmooth=inner[1])
         return mg.Multigrid(mg.Grid(pr), pr, walls=self.walls)
 
     def random_rhs(self, seed=0):

@@ -363,3 +363,25 @@ def test_mixed_solve_reports_initial_guess_residual(gcr):
     assert sm.converged and abs(sm.residual0 - want) <= 1e-12 * want
     s64 = g64.mgSolve(guess.copy(), f)
     assert s64.residual0 == -1.0
+
+
+@pytest.mark.parametrize("inner", [(3, 3), (1, 2)])
+def test_gcr_inner_cycle_smoothing_counts(inner):
+    """innerPreSmooth / innerPostSmooth set the FP32 preconditioner cycle only: the solve still reaches
+    the FP64 solution to the stop rule, and more inner smoothing never costs outer cycles."""
+    from saras_amd import _lib
+    pb = Problem(ind=(6, 6, 5), domain="PPN", mesh="UUD", semantics=po.SEM_PROJECTED, rel_tol=1e-10)
+    g64 = pb.gpu()
+    g22 = pb.gpu(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1)
+    gin = pb.gpu(precision=_lib.SR_PREC_MIXED_GCR, coarse_direct=1, inner=inner)
+    f = pb.random_rhs(23)
+    p64, p22, pin = (np.zeros(pb.shape) for _ in range(3))
+    s64, s22, sin_ = g64.mgSolve(p64, f), g22.mgSolve(p22, f), gin.mgSolve(pin, f)
+    assert s64.converged and s22.converged and sin_.converged and sin_.residual < 1e-10
+    if inner[0] + inner[1] > 4:
+        assert sin_.cycles <= s22.cycles
+    else:
+        assert sin_.cycles >= s22.cycles
+    a, b = pin[1:-1, 1:-1, 1:-1], p64[1:-1, 1:-1, 1:-1]
+    d = (a - a.mean()) - (b - b.mean())
+    assert np.max(np.abs(d)) < 1e-7 * np.max(np.abs(b - b.mean()))
