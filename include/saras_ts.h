@@ -56,6 +56,7 @@ typedef struct sr_ts_desc {
     int precision;                  /* pressure solve: SR_PREC_FP64, _MIXED or _MIXED_GCR          */
     int coarseDirect;               /* pressure solve: sr_mg_desc.coarseDirect                     */
     int innerPreSmooth, innerPostSmooth; /* pressure solve: sr_mg_desc.innerPreSmooth / _PostSmooth */
+    int padMode;                    /* pressure solve: sr_mg_desc.padMode                          */
 } sr_ts_desc;
 
 typedef struct sr_ts_stats {

@@ -58,7 +58,7 @@ class SrTsDesc(C.Structure):
         ("scalar", C.c_int), ("rbcType", C.c_int), ("Ra", C.c_double), ("Pr", C.c_double),
         ("semantics", C.c_int), ("relTolerance", C.c_double), ("precision", C.c_int),
         ("coarseDirect", C.c_int),
-        ("innerPreSmooth", C.c_int), ("innerPostSmooth", C.c_int),
+        ("innerPreSmooth", C.c_int), ("innerPostSmooth", C.c_int), ("padMode", C.c_int),
     ]
 
 

@@ -296,6 +296,7 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
     md.coarseDirect = d.coarseDirect;
     md.innerPreSmooth = d.innerPreSmooth;
     md.innerPostSmooth = d.innerPostSmooth;
+    md.padMode = d.padMode;
     md.mgTolerance = d.mgTolerance;
     md.relTolerance = d.relTolerance > 0.0 ? d.relTolerance : 1e-10;
     md.dXi = t->h[0]; md.dEt = t->h[1]; md.dZt = t->h[2];

@@ -22,7 +22,7 @@ def _ptr(a):
 def ts_desc(ind=(7, 7, 7), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1.0, 1.0, 1.0), Re=1600.0,
             dt=1e-3, cn_tol=1e-6, depth=None, vc_count=5, pre=4, post=4, res_type=1, mg_tol=1e-6, prob_type=2,
             scalar=False, rbc_type=2, Ra=1e7, Pr=1.0, semantics=_L.SR_SEM_REFERENCE, rel_tol=1e-10,
-            precision=_L.SR_PREC_FP64, coarse_direct=0, gs_smooth=1, inner=(0, 0)):
+            precision=_L.SR_PREC_FP64, coarse_direct=0, gs_smooth=1, inner=(0, 0), pad_mode=0):
     """The parser fields of a hydro (or, with scalar=True, scalar) run (input/parameters.yaml keys;
     defaults: the Taylor-Green case)."""
     d = _L.SrTsDesc()
@@ -30,6 +30,7 @@ def ts_desc(ind=(7, 7, 7), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=(1
     d.semantics, d.relTolerance, d.precision = semantics, rel_tol, precision
     d.coarseDirect = int(coarse_direct)
     d.innerPreSmooth, d.innerPostSmooth = (int(k) for k in inner)
+    d.padMode = int(pad_mode)
     d.xInd, d.yInd, d.zInd = ind
     d.Lx, d.Ly, d.Lz = L
     d.xGrid, d.yGrid, d.zGrid = (1 if c == "U" else 2 for c in mesh)

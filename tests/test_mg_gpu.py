@@ -23,6 +23,10 @@ CONFIGS = {
     "ppp_mixed": dict(ind=(4, 5, 3), domain="PPP", mesh="UUU", L=(6.283185307179586,) * 3),
     "nnn_neumann_stretched": dict(ind=(4, 4, 4), domain="NNN", mesh="DDD", beta=(1.2, 0.8, 1.5)),
     "ppn_tanh_rbc_shape": dict(ind=(5, 5, 4), domain="PPN", mesh="UUD"),
+    # residual pads after each direction's ghost rule (mirror at Neumann walls) instead of zero
+    "ppn_tanh_33_mirror_pads": dict(ind=(5, 5, 5), domain="PPN", mesh="UUD", pad_mode=1),
+    "nnn_neumann_stretched_mirror_pads": dict(ind=(4, 4, 4), domain="NNN", mesh="DDD", beta=(1.2, 0.8, 1.5),
+                                              pad_mode=1),
 }
 
 
@@ -383,5 +387,25 @@ def test_gcr_inner_cycle_smoothing_counts(inner):
     else:
         assert sin_.cycles >= s22.cycles
     a, b = pin[1:-1, 1:-1, 1:-1], p64[1:-1, 1:-1, 1:-1]
+    d = (a - a.mean()) - (b - b.mean())
+    assert np.max(np.abs(d)) < 1e-7 * np.max(np.abs(b - b.mean()))
+
+
+@pytest.mark.parametrize("gcr", [False, True], ids=["refinement", "gcr"])
+def test_mixed_precision_with_mirror_pads(gcr):
+    """padMode 1 changes the restriction only: the mixed solves still reach the FP64 solution, and on a
+    Neumann-walled case they need no more cycles than with the reference's zero pads."""
+    from saras_amd import _lib
+    prec = _lib.SR_PREC_MIXED_GCR if gcr else _lib.SR_PREC_MIXED
+    pb0 = Problem(ind=(6, 6, 5), domain="PPN", mesh="UUD", semantics=po.SEM_PROJECTED, rel_tol=1e-10)
+    pb1 = Problem(ind=(6, 6, 5), domain="PPN", mesh="UUD", semantics=po.SEM_PROJECTED, rel_tol=1e-10, pad_mode=1)
+    f = pb0.random_rhs(29)
+    p64, p0, p1 = (np.zeros(pb0.shape) for _ in range(3))
+    s64 = pb0.gpu().mgSolve(p64, f)
+    s0 = pb0.gpu(precision=prec, coarse_direct=int(gcr)).mgSolve(p0, f)
+    s1 = pb1.gpu(precision=prec, coarse_direct=int(gcr)).mgSolve(p1, f)
+    assert s64.converged and s0.converged and s1.converged and s1.residual < 1e-10
+    assert s1.cycles <= s0.cycles
+    a, b = p1[1:-1, 1:-1, 1:-1], p64[1:-1, 1:-1, 1:-1]
     d = (a - a.mean()) - (b - b.mean())
     assert np.max(np.abs(d)) < 1e-7 * np.max(np.abs(b - b.mean()))
