@@ -546,8 +546,11 @@ void sr::vcycle_fp64(sr_mg *m) {
             smooth_level(m, l + 1, d.preSmooth, 1, false, false);
         }
     }
+    // post-smoothing sweeps colour 0 first, which reads only colour-1 nodes: the correction of the
+    // colour-0 nodes would be overwritten unread (Gauss-Seidel with at least one sweep)
+    const int pcol = d.gsSmooth && d.postSmooth > 0 ? 1 : -1;
     for (int l = bottom - 1; l >= 0; l--) {
-        launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream);
+        launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream, pcol);
         smooth_level(m, l, d.postSmooth, l == 0 ? 0 : 1, false, false);
     }
 }
@@ -610,8 +613,10 @@ void sr::vcycle_fp32(sr_mg *m) {
             smooth32(m, l + 1, m->pre32, 1, true);
         }
     }
+    // colour 0 is swept first by the post-smoothing: its correction would be overwritten unread
+    const int pcol = m->post32 > 0 ? 1 : -1;
     for (int l = bottom - 1; l >= 0; l--) {
-        launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s);
+        launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s, pcol);
         smooth32(m, l, m->post32, l == 0 ? 0 : 1, false);
     }
 }

@@ -17,7 +17,9 @@ void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, c
                      hipStream_t s);
 template <typename T> void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s);
 template <typename T, typename TF>
-void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s);
+// color >= 0: that colour's nodes only (the up leg before red-black smoothing that sweeps the other
+// colour first); -1: every core node
+void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s, int color = -1);
 template <typename T> void launch_fill(T *a, long n, T v, hipStream_t s);
 template <typename T>
 void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int core, hipStream_t s);

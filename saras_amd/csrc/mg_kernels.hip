@@ -291,19 +291,21 @@ __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T 
 }
 
 // ------------------------------------------------------------------------------------------------
-// prolongation: fine u (+)= trilinear interpolation of coarse e (core nodes only)
+// prolongation: fine u (+)= trilinear interpolation of coarse e (core nodes only).  color >= 0
+// updates that colour's nodes only (geometry geo_sweep): a red-black sweep of the other colour
+// overwrites those nodes next without reading them, so their correction would be dead work.
 // ------------------------------------------------------------------------------------------------
 template <typename T, typename TF>
 __global__ void __launch_bounds__(256) k_prolong(Lay F, Lay Cl, Geo G, const T *__restrict__ e, TF *__restrict__ u,
-                                                 int accumulate) {
+                                                 int accumulate, int color) {
     unsigned ipu, rr;
     if (!geo_slot(G, xcd_block(), ipu, rr)) return;
     int ip = (int)ipu;
     unsigned jr = fdivu(rr, G.d1, G.inv1);
     unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = r2 >= (unsigned)F.kh ? 1 : 0;
-    int h = (int)r2 - c * F.kh;
+    int c = color >= 0 ? color : (r2 >= (unsigned)F.kh ? 1 : 0);
+    int h = color >= 0 ? (int)r2 : (int)r2 - c * F.kh;
     int kp = 2 * h + ((c + 1 + ip + jp) & 1);
     if (kp < 1 || kp > F.nz) return;
     int i = ip - 1, j = jp - 1, k = kp - 1;
@@ -1355,9 +1357,10 @@ void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t
 }
 
 template <typename T, typename TF>
-void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s) {
-    Geo G = geo_core(F);
-    hipLaunchKernelGGL((k_prolong<T, TF>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, e, u, accumulate);
+void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s, int color) {
+    Geo G = color >= 0 ? geo_sweep(F) : geo_core(F);
+    hipLaunchKernelGGL((k_prolong<T, TF>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, e, u, accumulate,
+                       color);
 }
 
 template <typename T>
@@ -1556,9 +1559,9 @@ template void launch_residual<float, float>(const Lay &, const float *, const fl
                                             const float *, const float *, hipStream_t);
 template void launch_residual<double, float>(const Lay &, const double *, const double *, float *,
                                              const double *, const double *, const double *, hipStream_t);
-template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t);
-template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t);
-template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t);
+template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t, int);
+template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t, int);
+template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t, int);
 template void launch_pack<double, double>(const Lay &, const double *, double *, int, hipStream_t);
 template void launch_pack<double, float>(const Lay &, const double *, float *, int, hipStream_t);
 template void launch_unpack<double, double>(const Lay &, const double *, double *, hipStream_t);
