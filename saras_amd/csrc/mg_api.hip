@@ -499,9 +499,9 @@ void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, b
     if (exact || !m->d.gsSmooth) launch_fill_ghosts<double>(V.L, m->lv[l].u, bc, s);
 }
 
-void sr::residual_level(sr_mg *m, int l) {
+void sr::residual_level(sr_mg *m, int l, int color) {
     Level &V = m->lv[l];
-    launch_residual<double, double>(V.L, V.u, V.f, V.t, V.c64[0], V.c64[1], V.c64[2], m->stream);
+    launch_residual<double, double>(V.L, V.u, V.f, V.t, V.c64[0], V.c64[1], V.c64[2], m->stream, color);
     launch_fill_ghosts<double>(V.L, V.t, tmp_bc(m), m->stream);
 }
 
@@ -530,9 +530,12 @@ void sr::vcycle_fp64(sr_mg *m) {
     int Lc = (m->coarse_enable && d.gsSmooth) ? m->coarse_first64 : (1 << 20);
     int bottom = Lc <= D ? Lc : D;   // last level handled by per-level launches + 1
     smooth_level(m, 0, d.preSmooth, 0, false, false, m->sw0, m->sw1);
+    // pre-smoothing ends on a colour-1 half sweep, after which the colour-1 residual is zero: only
+    // colour 0 is computed and restricted
+    const bool red0 = d.gsSmooth && d.preSmooth > 0;
     for (int l = 0; l < bottom; l++) {
-        residual_level(m, l);
-        launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream);
+        residual_level(m, l, red0 ? 0 : -1);
+        launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream, red0);
         if (l + 1 == Lc) {
             launch_coarse_cycle<double>(m->cdesc64, m->clds64, m->lv[Lc].u, m->lv[Lc].f, m->diters, m->stream);
         } else if (l + 1 == D) {
@@ -591,11 +594,13 @@ void sr::vcycle_fp32(sr_mg *m) {
     int Lc = m->coarse_enable ? m->coarse_first32 : (1 << 20);
     int bottom = Lc <= D ? Lc : D;
     smooth32(m, 0, m->pre32, 0, true, m->sw0, m->sw1);
+    // the pre-smoothing ends on colour 1: the colour-1 residual is zero, only colour 0 is computed
+    const bool red0 = m->pre32 > 0;
     for (int l = 0; l < bottom; l++) {
         Level &V = m->lv[l];
-        launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s);
+        launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s, red0 ? 0 : -1);
         launch_fill_ghosts<float>(V.L, V.t32, tmp_bc(m), s);
-        launch_restrict<float>(V.L, m->lv[l + 1].L, V.t32, m->lv[l + 1].f32, s);
+        launch_restrict<float>(V.L, m->lv[l + 1].L, V.t32, m->lv[l + 1].f32, s, red0);
         if (l + 1 == Lc) {
             launch_coarse_cycle<float>(m->cdesc32, m->clds32, m->lv[Lc].u32, m->lv[Lc].f32, m->diters, s);
         } else if (l + 1 == D) {

@@ -14,8 +14,11 @@ void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, c
                      const Bc &bc, bool zero_start, hipStream_t s);
 template <typename T, typename TO>
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
-                     hipStream_t s);
-template <typename T> void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s);
+                     hipStream_t s, int color = -1);
+// red0: t is zero on colour 1 and only its colour-0 entries are read (residual after a red-black
+// sweep ending on colour 1, computed with color = 0)
+template <typename T>
+void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0 = false);
 template <typename T, typename TF>
 // color >= 0: that colour's nodes only (the up leg before red-black smoothing that sweeps the other
 // colour first); -1: every core node

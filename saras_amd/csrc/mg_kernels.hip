@@ -241,18 +241,20 @@ __device__ inline T apply_lap(const Lay &L, const T *__restrict__ u, long row_c,
     return X[0] * xp + X[1] * xm + Y[0] * yp + Y[1] * ym + Z[0] * zp + Z[1] * zm - diag3<T>(X, Y, Z) * c;
 }
 
+// color >= 0: that colour's nodes only (geometry geo_sweep), the other colour's entries of t_out
+// are left as they are
 template <typename T, typename TO>
 __global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
                                                   TO *__restrict__ t_out, const T *__restrict__ cx,
-                                                  const T *__restrict__ cy, const T *__restrict__ cz) {
+                                                  const T *__restrict__ cy, const T *__restrict__ cz, int color) {
     unsigned ipu, rr;
     if (!geo_slot(G, xcd_block(), ipu, rr)) return;
     int ip = (int)ipu;
     unsigned jr = fdivu(rr, G.d1, G.inv1);
     unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = r2 >= (unsigned)L.kh ? 1 : 0;
-    int h = (int)r2 - c * L.kh;
+    int c = color >= 0 ? color : (r2 >= (unsigned)L.kh ? 1 : 0);
+    int h = color >= 0 ? (int)r2 : (int)r2 - c * L.kh;
     int kb = (c + 1 + ip + jp) & 1;
     int kp = 2 * h + kb;
     if (kp < 1 || kp > L.nz) return;
@@ -262,9 +264,12 @@ __global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restr
 }
 
 // ------------------------------------------------------------------------------------------------
-// restriction: coarse f(I,J,K) = full weighting of fine t around (2I, 2J, 2K), ghosts included
+// restriction: coarse f(I,J,K) = full weighting of fine t around (2I, 2J, 2K), ghosts included.
+// RED0: t vanishes on colour 1 (the residual right after a red-black sweep that ended on colour 1,
+// which leaves every colour-1 equation satisfied): the coincident node and the 12 edge points are
+// colour 0 (even offset sums), the 6 face and 8 vertex points colour 1, so only the former are read.
 // ------------------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool RED0>
 __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T *__restrict__ t, T *__restrict__ fc) {
     unsigned ipu, rr;
     if (!geo_slot(G, xcd_block(), ipu, rr)) return;
@@ -279,6 +284,13 @@ __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T 
     // fine padded index of the coincident node: logical 2i -> padded 2(ip-1)+1
     int fi = 2 * ip - 1, fj = 2 * jp - 1, fk = 2 * kp - 1;
 #define TT(a, b, e) t[lay_idx(F, fi + (a), fj + (b), fk + (e))]
+    if (RED0) {
+        T edgePoints = (TT(1, 1, 0) + TT(1, -1, 0) + TT(-1, -1, 0) + TT(-1, 1, 0) + TT(0, 1, 1) + TT(0, -1, 1) +
+                        TT(0, -1, -1) + TT(0, 1, -1) + TT(1, 0, 1) + TT(1, 0, -1) + TT(-1, 0, -1) + TT(-1, 0, 1)) *
+                       T(0.03125);
+        fc[lay_row(Cl, ip, jp, c) + h] = edgePoints + TT(0, 0, 0) * T(0.125);
+        return;
+    }
     T facePoints = (TT(1, 0, 0) + TT(-1, 0, 0) + TT(0, 1, 0) + TT(0, -1, 0) + TT(0, 0, 1) + TT(0, 0, -1)) * T(0.0625);
     T edgePoints = (TT(1, 1, 0) + TT(1, -1, 0) + TT(-1, -1, 0) + TT(-1, 1, 0) + TT(0, 1, 1) + TT(0, -1, 1) +
                     TT(0, -1, -1) + TT(0, 1, -1) + TT(1, 0, 1) + TT(1, 0, -1) + TT(-1, 0, -1) + TT(-1, 0, 1)) *
@@ -1345,15 +1357,18 @@ void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, c
 
 template <typename T, typename TO>
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
-                     hipStream_t s) {
-    Geo G = geo_core(L);
-    hipLaunchKernelGGL((k_residual<T, TO>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz);
+                     hipStream_t s, int color) {
+    Geo G = color >= 0 ? geo_sweep(L) : geo_core(L);
+    hipLaunchKernelGGL((k_residual<T, TO>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz, color);
 }
 
 template <typename T>
-void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s) {
+void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0) {
     Geo G = geo_core(Cl);
-    hipLaunchKernelGGL(k_restrict<T>, dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc);
+    if (red0)
+        hipLaunchKernelGGL((k_restrict<T, true>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc);
+    else
+        hipLaunchKernelGGL((k_restrict<T, false>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc);
 }
 
 template <typename T, typename TF>
@@ -1526,7 +1541,7 @@ int coarse_cycle_attr(size_t lds) {
     template void launch_fill_ghosts<T>(const Lay &, T *, const Bc &, hipStream_t);                        \
     template void launch_rb_sweep<T>(const Lay &, T *, const T *, const T *, const T *, const T *, int,     \
                                      const Bc &, bool, hipStream_t);                                        \
-    template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t);               \
+    template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t, bool);         \
     template void launch_fill<T>(T *, long, T, hipStream_t);                                               \
     template void launch_add_scalar<T>(const Lay &, T *, const double *, double, int, hipStream_t);        \
     template void launch_norm_res<T>(const Lay &, const T *, const T *, const T *, const T *, const T *,   \
@@ -1554,11 +1569,11 @@ template int coarse_cycle_attr<float>(size_t);
 #undef SR_INST
 
 template void launch_residual<double, double>(const Lay &, const double *, const double *, double *,
-                                              const double *, const double *, const double *, hipStream_t);
+                                              const double *, const double *, const double *, hipStream_t, int);
 template void launch_residual<float, float>(const Lay &, const float *, const float *, float *, const float *,
-                                            const float *, const float *, hipStream_t);
+                                            const float *, const float *, hipStream_t, int);
 template void launch_residual<double, float>(const Lay &, const double *, const double *, float *,
-                                             const double *, const double *, const double *, hipStream_t);
+                                             const double *, const double *, const double *, hipStream_t, int);
 template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t, int);
 template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t, int);
 template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t, int);

@@ -87,7 +87,7 @@ Bc tmp_bc(const sr_mg *m);
 Bc error_bc(const sr_mg *m, int l, int zeroBC);
 void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
                   hipEvent_t ea = nullptr, hipEvent_t eb = nullptr);
-void residual_level(sr_mg *m, int l);
+void residual_level(sr_mg *m, int l, int color = -1);
 void coarsest_level(sr_mg *m, int l, int zeroBC);
 void vcycle_fp64(sr_mg *m);
 void vcycle_fp32(sr_mg *m);
