@@ -641,9 +641,10 @@ int sr::ensure_fp32(sr_mg *m) {
     return SR_OK;
 }
 
-void sr::norm_level0(sr_mg *m) {
+void sr::norm_level0(sr_mg *m, int color) {
     Level &V = m->lv[0];
-    launch_norm_res<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part2, m->fterms, m->dres, m->stream);
+    launch_norm_res<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], m->part2, m->fterms, m->dres, m->stream,
+                            color);
 }
 
 void sr::f_terms_level0(sr_mg *m) {

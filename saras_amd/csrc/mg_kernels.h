@@ -43,7 +43,7 @@ template <typename T> void launch_f_terms(const Lay &L, const T *f, double *part
 // out[1] and out[3] are copied from fterms
 template <typename T>
 void launch_norm_res(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part2,
-                     const double *fterms, double *out, hipStream_t s);
+                     const double *fterms, double *out, hipStream_t s, int color = -1);
 // iterative-refinement step: e == nullptr computes r = f - L u_in only; otherwise u_out = u_in + e
 // (core and face ghosts) and r = f - L u_out.  r is stored in FP32; out[0..3] as launch_norm_res.
 void launch_outer_residual(const Lay &L, const double *u_in, const float *e, const double *f, double *u_out,

@@ -469,16 +469,17 @@ __device__ inline void block_reduce2(double s, double m, double *part) {
 }
 
 // Reduction kernels visit the core nodes of four consecutive x-planes per thread (logical planes
-// are plane groups: geo_core4), so one block reduction serves up to 1024 nodes.
+// are plane groups: geo_core4), so one block reduction serves up to 1024 nodes.  color >= 0 visits
+// that colour's nodes only (geometry geo_core4c).
 template <typename F>
-__device__ inline void core4_visit(const Lay &L, const Geo &G, F &&fn) {
+__device__ inline void core4_visit(const Lay &L, const Geo &G, F &&fn, int color = -1) {
     unsigned gu, rr;
     if (!geo_slot(G, xcd_block(), gu, rr)) return;
     unsigned jr = fdivu(rr, G.d1, G.inv1);
     unsigned r2 = rr - jr * G.d1;
     int jp = 1 + (int)jr;
-    int c = r2 >= (unsigned)L.kh ? 1 : 0;
-    int h = (int)r2 - c * L.kh;
+    int c = color >= 0 ? color : (r2 >= (unsigned)L.kh ? 1 : 0);
+    int h = color >= 0 ? (int)r2 : (int)r2 - c * L.kh;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         int ip = 1 + 4 * (int)gu + q;
@@ -489,11 +490,12 @@ __device__ inline void core4_visit(const Lay &L, const Geo &G, F &&fn) {
 }
 
 // per-cycle residual norm: sum |Lu - f|^2 and max |Lu - f| per block.  The right-hand side terms of computeError do not change during a solve
-// and are reduced once per solve (k_f_terms).
+// and are reduced once per solve (k_f_terms).  color >= 0: over that colour only (after a red-black
+// sweep of the other colour, whose residual is then zero).
 template <typename T>
 __global__ void __launch_bounds__(256) k_norm_res(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
                                                   const T *__restrict__ cx, const T *__restrict__ cy,
-                                                  const T *__restrict__ cz, double *__restrict__ part) {
+                                                  const T *__restrict__ cz, double *__restrict__ part, int color) {
     double s = 0.0, mm = 0.0;
     core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
         long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
@@ -501,7 +503,7 @@ __global__ void __launch_bounds__(256) k_norm_res(Lay L, Geo G, const T *__restr
         double v = fabs(lu - (double)f[rc + h]);
         s += v * v;
         mm = fmax(mm, v);
-    });
+    }, color);
     block_reduce2(s, mm, part);
 }
 
@@ -1330,6 +1332,8 @@ static Geo geo_sweep(const Lay &L) { return geo_planes((unsigned)(L.ny * L.kh), 
 static Geo geo_core(const Lay &L) { return geo_planes((unsigned)(L.ny * 2 * L.kh), 1, L.nx, 2 * L.kh); }
 // both colours of the core nodes, four x-planes per thread (core4_visit)
 static Geo geo_core4(const Lay &L) { return geo_planes((unsigned)(L.ny * 2 * L.kh), 0, (L.nx + 3) / 4, 2 * L.kh); }
+// one colour of geo_core4
+static Geo geo_core4c(const Lay &L) { return geo_planes((unsigned)(L.ny * L.kh), 0, (L.nx + 3) / 4, L.kh); }
 // every stored slot of the padded level
 static Geo geo_all(const Lay &L) { return geo_planes((unsigned)((L.ny + 2) * 2 * L.khs), 0, L.nx + 2, 2 * L.khs); }
 static const unsigned REDUCE_BLOCKS = 2048;
@@ -1473,10 +1477,10 @@ int norm_res_parts(const Lay &L) { return (int)geo_core4(L).total_blocks; }
 
 template <typename T>
 void launch_norm_res(const Lay &L, const T *u, const T *f, const T *cx, const T *cy, const T *cz, double *part2,
-                     const double *fterms, double *out, hipStream_t s) {
-    Geo G = geo_core4(L);
+                     const double *fterms, double *out, hipStream_t s, int color) {
+    Geo G = color >= 0 ? geo_core4c(L) : geo_core4(L);
     int nb = (int)G.total_blocks;
-    hipLaunchKernelGGL(k_norm_res<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part2);
+    hipLaunchKernelGGL(k_norm_res<T>, dim3(nb), dim3(256), 0, s, L, G, u, f, cx, cy, cz, part2, color);
     hipLaunchKernelGGL(k_norm_combine, dim3(1), dim3(1024), 0, s, part2, nb, fterms, out);
 }
 
@@ -1545,7 +1549,7 @@ int coarse_cycle_attr(size_t lds) {
     template void launch_fill<T>(T *, long, T, hipStream_t);                                               \
     template void launch_add_scalar<T>(const Lay &, T *, const double *, double, int, hipStream_t);        \
     template void launch_norm_res<T>(const Lay &, const T *, const T *, const T *, const T *, const T *,   \
-                                     double *, const double *, double *, hipStream_t);                      \
+                                     double *, const double *, double *, hipStream_t, int);                 \
     template void launch_f_terms<T>(const Lay &, const T *, double *, double *, hipStream_t);               \
     template void launch_coarsest_direct<T>(const Lay &, T *, const T *, const T *, int, const Bc &, int *,    \
                                             hipStream_t);                                                  \

@@ -81,7 +81,9 @@ static int norm_order(const sr_mg *m) {
 // one FP64 cycle plus its norm and the read-back of the reduced quantities
 static void cycle_fp64(sr_mg *m) {
     vcycle_fp64(m);
-    norm_level0(m);
+    // a Gauss-Seidel cycle ends on a colour-1 half sweep of level 0, which leaves the colour-1
+    // residual zero (up to the rounding of f - Lu): the norm is taken over colour 0
+    norm_level0(m, m->d.gsSmooth && m->d.postSmooth > 0 ? 0 : -1);
     (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
 }
 

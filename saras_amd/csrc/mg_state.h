@@ -92,7 +92,7 @@ void coarsest_level(sr_mg *m, int l, int zeroBC);
 void vcycle_fp64(sr_mg *m);
 void vcycle_fp32(sr_mg *m);
 int ensure_fp32(sr_mg *m);
-void norm_level0(sr_mg *m);       // needs the right-hand side terms of f_terms_level0
+void norm_level0(sr_mg *m, int color = -1);   // needs the right-hand side terms of f_terms_level0
 void f_terms_level0(sr_mg *m);
 double error_from(const sr_mg *m, const double *q, int norm);
 
