@@ -482,17 +482,17 @@ static int lin_fit(sr_guess *g, sr_mg *m, const double *rhs) {
     hipStream_t s = g->s;
     const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
     if (singular_mg(m)) {
-        hipLaunchKernelGGL(k_wsum, dim3(kRedBlocks), dim3(256), 0, s, f, rhs, m->wvec[0], m->wvec[1], m->wvec[2],
+        hipLaunchKernelGGL(k_wsum, dim3(kLinBlocks), dim3(256), 0, s, f, rhs, m->wvec[0], m->wvec[1], m->wvec[2],
                            g->lpart);
-        hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kRedBlocks, 1, g->lsum);
+        hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kLinBlocks, 1, g->lsum);
         hipLaunchKernelGGL(k_project_rhs, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, rhs, g->lsum, 1.0 / m->wsum,
                            g->fcur);
     } else {
         hipLaunchKernelGGL(k_project_rhs, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, rhs, (const double *)nullptr,
                            0.0, g->fcur);
     }
-    hipLaunchKernelGGL(k_dots, dim3(kRedBlocks), dim3(256), 0, s, f, g->fcur, ptrs(g->hf, g->hn), g->hn, g->lpart);
-    hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kRedBlocks, 9, g->lsum);
+    hipLaunchKernelGGL(k_dots, dim3(kLinBlocks), dim3(256), 0, s, f, g->fcur, ptrs(g->hf, g->hn), g->hn, g->lpart);
+    hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kLinBlocks, 9, g->lsum);
     HIPCHK(hipMemcpyAsync(g->lsum_h, g->lsum, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
@@ -510,7 +510,7 @@ static int lin_apply(sr_guess *g, double *out) {
     const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
     Coef8 c{};
     for (int i = 0; i < g->hn; i++) c.c[i] = g->hc[i];
-    hipLaunchKernelGGL(k_lincomb, dim3(kRedBlocks), dim3(256), 0, g->s, f, (const double *)nullptr, 0.0,
+    hipLaunchKernelGGL(k_lincomb, dim3(kLinBlocks), dim3(256), 0, g->s, f, (const double *)nullptr, 0.0,
                        ptrs(g->hp, g->hn), c, g->hn, out, (double *)nullptr);
     HIPCHK(hipGetLastError());
     return SR_OK;

@@ -125,7 +125,7 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
         }
         RETCHK(dalloc(g, &g->fcur, g->fine_pad));
         RETCHK(dalloc(g, &g->xlin, g->fine_pad));
-        RETCHK(dalloc(g, &g->lpart, 1024L * 16));
+        RETCHK(dalloc(g, &g->lpart, (long)kLinBlocks * 16));
         RETCHK(dalloc(g, &g->lsum, 16));
         HIPCHK(hipHostMalloc((void **)&g->lsum_h, sizeof(double) * 16, hipHostMallocDefault));
     }

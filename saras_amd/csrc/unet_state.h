@@ -27,6 +27,10 @@ enum { X0, A0, S0, X1, A1, S1, X2, A2, S2, X3, A3, BB, U2, G2, U1, G1, U0, G0 };
 
 }  // namespace sr
 
+// workgroups of the linear predictor's streaming kernels over the solver grid (dots, combination):
+// 16 waves per CU keep enough FP64 loads in flight
+constexpr int kLinBlocks = 4096;
+
 struct sr_guess {
     sr_guess_desc d{};
     sr::NetGrid lv[4]{};
@@ -66,7 +70,7 @@ struct sr_guess {
     double hc[SR_GUESS_MAX_HISTORY] = {};      // least-squares coefficients of the current step
     double yy = 0.0, fit_r2 = 0.0;             // ||fcur||^2 and the fit's squared residual
     bool have_fit = false;                     // fcur / hd / hc belong to the current step
-    double *lpart = nullptr, *lsum = nullptr, *lsum_h = nullptr;
+    double *lpart = nullptr, *lsum = nullptr, *lsum_h = nullptr;   // lpart: kLinBlocks x 16 partials
 };
 
 namespace sr {
