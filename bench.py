@@ -532,10 +532,10 @@ def main():
     plan = {"coarse_cycle_first_level_fp64": solver.info(0), "coarse_cycle_lds_bytes_fp64": solver.info(2),
             "lds_limit_per_workgroup": solver.info(4)}
     solver.close()
-    # whole-cycle algorithmic traffic (113.5 B per node per level: the residual on colour 0 only, 16 B,
-    # and the prolongation onto colour 1 only, 8.5 B; + 16 B per finest node for the norm)
+    # whole-cycle algorithmic traffic (114 B per node per level: the residual on colour 0 only, 16 B,
+    # and the prolongation onto colour 1 only, 9 B; + 12 B per finest node for the colour-0 norm)
     n_levels = [((1 << (8 - l)) + 1) ** 3 for l in range(p.vcDepth + 1)]
-    cyc_bytes = 113.5 * sum(n_levels[:-1]) + 16.0 * n_levels[0]
+    cyc_bytes = 114.0 * sum(n_levels[:-1]) + 12.0 * n_levels[0]
     solve_gbs = cyc_bytes * np.mean(cycles) / (ms * 1e-3) / 1e9
 
     # mixed precision: FP32 V(2,2) cycles under FP64 iterative refinement, same stop rule
