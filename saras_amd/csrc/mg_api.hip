@@ -480,6 +480,8 @@ void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, b
     hipStream_t s = m->stream;
     if (m->d.gsSmooth) {
         if (!zero_start) launch_fill_ghosts<double>(V.L, V.u, bc, s);
+        // a zero start without sweeps still starts from zero (poisson.cc clears the level)
+        if (zero_start && count == 0) launch_fill<double>(V.u, lay_size(V.L), 0.0, s);
         if (ea) mark(m, ea);
         for (int n = 0; n < count; n++) {
             launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 0, bc, zero_start && n == 0, s);
@@ -578,6 +580,8 @@ static void smooth32(sr_mg *m, int l, int count, int zeroBC, bool zero_start, hi
     if (!zs) {
         if (zero_start) launch_fill<float>(V.u32, lay_size(V.L), 0.0f, s);
         launch_fill_ghosts<float>(V.L, V.u32, bc, s);
+    } else if (count == 0) {
+        launch_fill<float>(V.u32, lay_size(V.L), 0.0f, s);   // a zero start without sweeps
     }
     if (ea) mark(m, ea);
     for (int n = 0; n < count; n++) {

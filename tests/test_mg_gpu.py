@@ -128,7 +128,20 @@ def test_coarsest_solve_and_compute_error(cfg):
 
 @pytest.mark.parametrize("cfg", list(CONFIGS))
 def test_vcycle_iterates(cfg):
-    pb, o, g = _pair(cfg)
+    _vcycle_iterates(*_pair(cfg))
+
+
+@pytest.mark.parametrize("pre,post", [(1, 0), (0, 1), (1, 1), (3, 2)])
+def test_vcycle_iterates_smoothing_counts(pre, post):
+    """The cycle restricts only the colour-0 residual after a red-black pre-smoothing (the colour-1
+    residual is then zero) and prolongs only onto colour 1 before a post-smoothing (colour 0 is then
+    overwritten unread); with a count of 0 the full residual and prolongation run.  Iterates match
+    the oracle's full operators for every combination."""
+    for cfg in ("ppn_tanh_33", "nnn_dirichlet_17"):
+        _vcycle_iterates(*_pair(cfg, pre=pre, post=post))
+
+
+def _vcycle_iterates(pb, o, g):
     rng = np.random.default_rng(7)
     P0 = rng.standard_normal(pb.shape)
     F0 = pb.random_rhs(8)
