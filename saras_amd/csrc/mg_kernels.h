@@ -54,8 +54,8 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
 // 5 doubles per block of norm_res_parts), then the step x_out = x_in + alpha (e - beta z),
 // z_next = e - beta z (face-ghost images by bc and ebc); the FP64 residual of x_out is then
 // launch_outer_residual's
-void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const double *cx,
-                     const double *cy, const double *cz, double *part5, double *coef, hipStream_t s);
+void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const float *cx,
+                     const float *cy, const float *cz, double *part5, double *coef, hipStream_t s);
 void launch_gcr_step(const Lay &L, const double *x_in, const float *e, const float *z, const double *coef,
                      double *x_out, float *z_next, const Bc &bc, const Bc &ebc, hipStream_t s);
 // exact coarsest solve u = P f with the dense inverse (transposed, nc x nc), ghosts filled by bc

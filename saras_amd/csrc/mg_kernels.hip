@@ -793,26 +793,26 @@ __global__ void __launch_bounds__(256) k_outer_residual(Lay L, Geo G, const doub
 // The directions and the inner products use FP32 data (they only steer the step); the iterate and
 // its residual are FP64, so the attained accuracy is that of the FP64 residual.
 // ------------------------------------------------------------------------------------------------
-template <typename TA>
-__device__ inline double lap_f(const Lay &L, const TA *__restrict__ a, long rc, long ro, int h, int kb,
-                               const double *X, const double *Y, const double *Z) {
+// per-block sums (r q, q q, q p, r p, p p), written at the hardware block index.  q and p are
+// formed in FP32 with the FP32 operator of the inner cycle (they only steer the step; the iterate's
+// residual is recomputed in FP64), the products are summed in FP64.
+__device__ inline float lap32(const Lay &L, const float *__restrict__ a, long rc, long ro, int h, int kb,
+                              const float *X, const float *Y, const float *Z) {
     const long sx = 2L * (L.ny + 2) * L.khs, sy = 2L * L.khs;
-    return X[0] * (double)a[ro + sx + h] + X[1] * (double)a[ro - sx + h] + Y[0] * (double)a[ro + sy + h] +
-           Y[1] * (double)a[ro - sy + h] + Z[0] * (double)a[ro + h + kb] + Z[1] * (double)a[ro + h - 1 + kb] -
-           diag3<double>(X, Y, Z) * (double)a[rc + h];
+    return X[0] * a[ro + sx + h] + X[1] * a[ro - sx + h] + Y[0] * a[ro + sy + h] + Y[1] * a[ro - sy + h] +
+           Z[0] * a[ro + h + kb] + Z[1] * a[ro + h - 1 + kb] - diag3<float>(X, Y, Z) * a[rc + h];
 }
 
-// per-block sums (r q, q q, q p, r p, p p), written at the hardware block index
 __global__ void __launch_bounds__(256) k_gcr_dots(Lay L, Geo G, const float *__restrict__ e,
                                                   const float *__restrict__ z, const float *__restrict__ r32,
-                                                  const double *__restrict__ cx, const double *__restrict__ cy,
-                                                  const double *__restrict__ cz, double *__restrict__ part) {
+                                                  const float *__restrict__ cx, const float *__restrict__ cy,
+                                                  const float *__restrict__ cz, double *__restrict__ part) {
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
         long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
-        const double *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
-        double q = lap_f<float>(L, e, rc, ro, h, kb, X, Y, Z);
-        double p = lap_f<float>(L, z, rc, ro, h, kb, X, Y, Z);
+        const float *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        double q = (double)lap32(L, e, rc, ro, h, kb, X, Y, Z);
+        double p = (double)lap32(L, z, rc, ro, h, kb, X, Y, Z);
         double r = (double)r32[rc + h];
         s[0] += r * q;
         s[1] += q * q;
@@ -1458,8 +1458,8 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
     hipLaunchKernelGGL(k_norm_combine, dim3(1), dim3(1024), 0, s, part2, nb, fterms, out);
 }
 
-void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const double *cx,
-                     const double *cy, const double *cz, double *part5, double *coef, hipStream_t s) {
+void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const float *cx,
+                     const float *cy, const float *cz, double *part5, double *coef, hipStream_t s) {
     Geo G = geo_core4(L);
     int nb = (int)G.total_blocks;
     hipLaunchKernelGGL(k_gcr_dots, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);

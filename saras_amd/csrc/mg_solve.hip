@@ -262,7 +262,7 @@ static void gcr_body(sr_mg *m, int cur) {
     const float *z = m->zdir[cur];
     float *z_next = m->zdir[cur ^ 1];
     vcycle_fp32(m);
-    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c64[0], V.c64[1], V.c64[2], m->gpart, m->gcoef, s);
+    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c32[0], V.c32[1], V.c32[2], m->gpart, m->gcoef, s);
     launch_gcr_step(V.L, x_in, V.u32, z, m->gcoef, x_out, z_next, pressure_bc(m, 0, 0), error_bc(m, 0, 0), s);
     launch_outer_residual(V.L, x_out, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
                           pressure_bc(m, 0, 0), m->part2, m->fterms, m->dres, s);
