@@ -43,16 +43,8 @@ def build():
 _libs = {}
 
 
-def lib(fast=False):
-    name = "liboracle_fast.so" if fast else "liboracle.so"
-    if name in _libs:
-        return _libs[name]
-    path = os.path.join(_BUILD, name)
-    srcs = [os.path.join(os.path.dirname(os.path.abspath(__file__)), f)
-            for f in ("mg_oracle.c", "field_oracle.c", "oracle.h")]
-    if not os.path.exists(path) or any(os.path.getmtime(f) > os.path.getmtime(path) for f in srcs if os.path.exists(f)):
-        build()
-    L = C.CDLL(path)
+def _bind(L):
+    """argtypes of the multigrid entry points (oracle.h)."""
     dp = C.POINTER(C.c_double)
     vp = C.c_void_p
     L.or_grid_staggr.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, dp, dp]
@@ -82,8 +74,26 @@ def lib(fast=False):
     L.or_mg_coef.argtypes = [vp, C.c_int, C.c_int]
     L.or_mg_max_count.restype = C.c_int
     L.or_mg_max_count.argtypes = [vp]
-    _libs[name] = L
     return L
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(_BUILD, name)
+    srcs = [os.path.join(os.path.dirname(os.path.abspath(__file__)), f)
+            for f in ("mg_oracle.c", "field_oracle.c", "oracle.h")]
+    if not os.path.exists(path) or any(os.path.getmtime(f) > os.path.getmtime(path) for f in srcs if os.path.exists(f)):
+        build()
+    _libs[name] = _bind(C.CDLL(path))
+    return _libs[name]
+
+
+def load(path):
+    """A separately built copy of the oracle (e.g. a deliberately mutated one in a test), bound
+    like lib(); pass it as `library=` to OracleMG / OracleTS."""
+    return _bind(C.CDLL(os.path.abspath(path)))
 
 
 def _ptr(a):
@@ -106,8 +116,8 @@ def grid_axis(n, length=1.0, grid_type=1, beta=1.0, kind="staggr", fast=False):
 class OracleMG:
     """The restated multigrid_d3 object: level hierarchies plus its protected operators."""
 
-    def __init__(self, params, metrics, walls=None, fast=False):
-        self.L = lib(fast)
+    def __init__(self, params, metrics, walls=None, fast=False, library=None):
+        self.L = library if library is not None else lib(fast)
         self.params = params
         self._keep = []
         ms = []
@@ -306,8 +316,8 @@ def ts_params(ind=(5, 5, 5), L=(2 * np.pi,) * 3, domain="PPP", mesh="UUU", beta=
 class OracleTS:
     """The restated hydro_d3 time step (fields, stencils, Jacobi viscous solve, projection)."""
 
-    def __init__(self, params, fast=False):
-        self.L = lib(fast)
+    def __init__(self, params, fast=False, library=None):
+        self.L = library if library is not None else lib(fast)
         _ts_bind(self.L)
         self.params = params
         self.h = self.L.or_ts_create(C.byref(params))

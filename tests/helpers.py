@@ -33,7 +33,7 @@ class Problem:
         self.shape = tuple(k + 2 for k in self.n)
 
     # ---- oracle ----
-    def oracle(self, smoother=po.SMOOTH_RB):
+    def oracle(self, smoother=po.SMOOTH_RB, library=None):
         ax = axes_oracle(self.n, self.L, self.gtype, self.beta)
         p = po.Params()
         p.xInd, p.yInd, p.zInd = self.ind
@@ -47,7 +47,7 @@ class Problem:
         p.mgTolerance, p.relTolerance = self.mg_tol, self.rel_tol
         p.dXi, p.dEt, p.dZt = (1.0 / (k - 1) for k in self.n)
         p.padMode, p.coarseProject = self.pad_mode, self.coarse_project
-        return po.OracleMG(p, metrics_from_axes(ax), walls=self.walls)
+        return po.OracleMG(p, metrics_from_axes(ax), walls=self.walls, library=library)
 
     # ---- HIP path ----
     def gpu(self, precision=0, coarse_direct=0, inner=(0, 0)):

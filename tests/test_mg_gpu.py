@@ -181,9 +181,10 @@ def test_mgsolve_matches_oracle(cfg, sem):
     assert np.max(np.abs(hg - hist)) <= 1e-9 * np.max(np.abs(hist))
     ok, err, scale = rel_close(pg, pe, RTOL * 10)
     assert ok, f"solution {err:.3e} / {scale:.3e}"
-    # The restated reference cycle diverges on fully periodic domains whose grid spacings differ
-    # per direction (ppp_mixed: factor ~2 per cycle at this depth, in the oracle as on the device),
-    # so that case checks parity of the iterates only.
+    # The reference's V(2,2) cycle diverges on z-periodic grids with hz = 2 hx (ppp_mixed: factor
+    # ~2 per cycle at this depth, in the oracle as on the device, lexicographic order included;
+    # test_oracle_pins.py shows the zero z pads and the weak z coupling behind it), so that case
+    # checks parity of the iterates only.
     if sem == po.SEM_PROJECTED and cfg != "ppp_mixed":
         assert st.converged == 1 and hist[-1] < 1e-10
 
