@@ -36,6 +36,25 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_BYTES = {"k_rb_sweep<float> 257x257x257": 124.4e6, "k_rb_sweep<double> 257x257x257": 229.0e6}
 
 
+def host_info():
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process may run on, and the
+    CPU model string."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = None
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
 def cpu_baseline(p, g, f, max_cycles, guess=None):
     """Oracle restatement of the reference algorithm (lexicographic Gauss-Seidel, one thread,
     -Ofast build like the reference's release flags) on the same workload and stop rule, from a
@@ -134,7 +153,7 @@ def mixed_seeds_bench(args):
         oracle = {"seed": 0, "cycles": len(hist), "final_rel_residual": float(hist[-1]),
                   **{f"{n}_max_abs_diff_vs_oracle": float(np.max(np.abs(sols0[n] - ref))) for n in sols0},
                   "solution_max_abs": scale}
-        cpu = {"value": round(cpu_s * 1e3, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+        cpu = {"value": round(cpu_s * 1e3, 1), "unit": "ms/solve", "cores": 1, "kind": "port", "host": host_info(),
                "sample": "the seed-0 solve by the oracle's FP64 multigrid (red-black V(2,2), -Ofast, 1 thread)"}
     keys = ("fp64_ms", "mixed_ms", "gcr_ms", "fp64_cycles", "mixed_cycles", "gcr_cycles")
     summ = {k + "_mean": float(np.mean([r[k] for r in rows])) for k in keys}
@@ -195,7 +214,7 @@ def tgv_bench(args):
             o.time_advance()
         c1 = time.perf_counter()
         o.close()
-        cpu = {"value": round((c1 - c0) * 1e3 / n_cpu, 1), "unit": "ms/step", "cores": 1, "kind": "port",
+        cpu = {"value": round((c1 - c0) * 1e3 / n_cpu, 1), "unit": "ms/step", "cores": 1, "kind": "port", "host": host_info(),
                "sample": f"{n_cpu} time steps of the same 129^3 Taylor-Green case by the oracle's restatement "
                          "of the reference step (lexicographic GS multigrid, -Ofast, 1 thread)"}
         if args.warmup == 0:
@@ -253,19 +272,24 @@ def rbc_bench(args):
                               **kw))
     run.attach_guess(net, train=True)
     run.initRBC(seed=0, amp=1e-3)
-    log(f"training over {args.warmup} steps")
+    log(f"training over {args.train_steps} steps")
     t0 = time.perf_counter()
     train_loss = []
-    for _ in range(args.warmup):
+    for i in range(args.train_steps):
         st = run.timeAdvance()
         if st.train_loss >= 0:
             train_loss.append(st.train_loss)
+        if (i + 1) % 100 == 0:
+            log(f"trained {i + 1} steps")
     train_s = time.perf_counter() - t0
     snap = {w: run.download(w) for w in (ts.VX, ts.VY, ts.VZ, ts.P, ts.PP, ts.T)}
-    # (a) learned + mixed, frozen network (used adaptively), from the trained state
-    log("timed learned + mixed steps")
+    # (a) learned + mixed, frozen network (used adaptively), from the trained state: W untimed
+    # steps, then K timed ones
+    log("learned + mixed steps")
     run.attach_guess(net, train=False)
     net.set_policy(2)
+    for _ in range(args.warmup):
+        run.timeAdvance()
     rec_l = [run.timeAdvance() for _ in range(args.steps)]
     learned = {"ms_per_solve": float(np.mean([r.solve_ms + r.guess_ms for r in rec_l])),
                "cycles_mean": float(np.mean([r.mg_cycles for r in rec_l])),
@@ -297,6 +321,8 @@ def rbc_bench(args):
         h.initRBC(seed=0, amp=1e-3)
         for w, a in snap.items():
             h.upload(w, a)
+        for _ in range(args.warmup):
+            h.timeAdvance()
         rec = [h.timeAdvance() for _ in range(args.steps)]
         out[name] = {"ms_per_solve": float(np.mean([r.solve_ms for r in rec])),
                      "cycles_mean": float(np.mean([r.mg_cycles for r in rec])),
@@ -304,6 +330,13 @@ def rbc_bench(args):
         sols[name] = h.download(ts.P)
         h.close()
     net.close()
+    # algorithmic bytes of one FP64 V(2,2) cycle: 114 B per node on every level above the coarsest
+    # (four sweeps of two half sweeps, 96 B; residual read on colour 0 with its restriction, 8 B, and
+    # the coarse right-hand side written, 1 B; prolongation onto colour 1, 9 B) + 12 B per finest
+    # node for the colour-0 norm
+    depth = nz_ind - 1
+    n_levels = [((n[0] - 1 >> l) + 1) * ((n[1] - 1 >> l) + 1) * ((n[2] - 1 >> l) + 1) for l in range(depth + 1)]
+    cyc_bytes = 114.0 * sum(n_levels[:-1]) + 12.0 * n_levels[0]
     a_, b_ = sol_l[1:-1, 1:-1, 1:-1], sols["fp64_warm"][1:-1, 1:-1, 1:-1]
     dP = (a_ - a_.mean()) - (b_ - b_.mean())
     base = out["fp64_warm"]["ms_per_solve"]
@@ -323,7 +356,7 @@ def rbc_bench(args):
         log("CPU baseline solve")
         g = mg.Grid(p)
         cpu_ms, cpu_cycles, cpu_res = cpu_baseline(p, g, cpu_rhs, args.cpu_cycles, guess=cpu_guess)
-        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port", "host": host_info(),
                "sample": f"one pressure solve of this workload (the last timed step's right-hand side, warm-started "
                          f"from the solution before it, as the reference's mgSolve is) by the oracle's restatement "
                          f"of the reference algorithm (lexicographic GS V(2,2), -Ofast, 1 thread): {cpu_cycles} "
@@ -343,10 +376,17 @@ def rbc_bench(args):
                    "mixed_solver": f"FP32 V({inner[0]},{inner[1]}) cycle preconditioning flexible GCR(1) under the FP64 residual, "
                                    "exact coarsest level, "
                                    + ("mirror residual pads at the walls" if pads else "the reference's residual pads"),
+                   "train_steps": args.train_steps,
+                   # the two ratios are co-equal: the first against the reference's algorithm (its zero
+                   # residual pads), the second against plain FP64 given the same mirror pads as the mixed
+                   # solves, which isolates what the guess and the mixed-precision solver are worth
                    "speedup_vs_fp64_warm": round(base / learned["ms_per_solve"], 3),
                    **({"speedup_vs_fp64_warm_mirror_pads":
                        round(out["fp64_warm_mirror_pads"]["ms_per_solve"] / learned["ms_per_solve"], 3)}
                       if pads else {}),
+                   "fp64_cycle_bytes_algorithmic": cyc_bytes,
+                   "fp64_warm_solve_GBs_algorithmic": round(cyc_bytes * out["fp64_warm"]["cycles_mean"] /
+                                                            (base * 1e-3) / 1e9, 1),
                    "pressure_max_abs_diff_vs_fp64_after_timed_steps": float(np.max(np.abs(dP))),
                    "pressure_rel_l2_diff_vs_fp64_after_timed_steps": float(np.linalg.norm(dP) /
                                                                           np.linalg.norm(b_ - b_.mean())),
@@ -425,7 +465,7 @@ def seq_bench(args):
     if not args.no_cpu:
         pc, gc, fc = workloads.stretched_poisson(ind=8, beta=1.0, seed=0)
         cpu_ms, cpu_cycles, cpu_res = cpu_baseline(pc, gc, seq.rhs(W), args.cpu_cycles)
-        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port", "host": host_info(),
                "sample": f"one solve (right-hand side of step {W}, zero guess) of the same 257^3 sequence by the "
                          f"oracle's restatement of the reference algorithm (lexicographic GS V(2,2), -Ofast, "
                          f"1 thread): {cpu_cycles} cycles to rel. residual {cpu_res:.2e}"}
@@ -458,8 +498,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps (default 30 for the sequence and convection workloads: the learned "
-                         "guess trains over them; 2 otherwise)")
+                    help="untimed steps before the timed ones (default 30 for the sequence workload, where the "
+                         "learned guess trains over them; 2 otherwise)")
+    ap.add_argument("--train-steps", type=int, default=500,
+                    help="convection workloads: time steps over which the learned guess trains online before the "
+                         "warm-up and timed steps of every leg (500: the first 500 steps of the convection run)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cycles", type=int, default=60)
     ap.add_argument("--cpu-steps", type=int, default=2,
@@ -478,7 +521,7 @@ def main():
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
     if args.warmup is None:
-        args.warmup = 30 if args.workload in ("seq257", "rbc257", "rbc257x129") else 2
+        args.warmup = 30 if args.workload == "seq257" else 2
     if args.workload == "seq257":
         seq_bench(args)
         return
@@ -573,7 +616,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu_ms, cpu_cycles, cpu_res = cpu_baseline(p, g, f, args.cpu_cycles)
-        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port", "host": host_info(),
                "sample": f"one full solve of the same 257^3 workload by the oracle's restatement of the "
                          f"reference algorithm (lexicographic GS V(2,2), -Ofast, 1 thread): {cpu_cycles} cycles "
                          f"to rel. residual {cpu_res:.2e}"}

@@ -71,6 +71,10 @@ typedef struct sr_ts_stats {
     double rel_guess;               /* relative residual of the guess handed to the solve          */
     double train_loss;              /* sr_guess_train's relative loss, -1 when not training        */
     double sweep0_ms;               /* sr_mg_stats.sweep0_ms of the pressure solve                 */
+    int mg_converged;               /* 1: the pressure solve met its stop rule; 0: it ran vcCount  */
+                                    /* cycles without meeting it and the step went on with the last */
+                                    /* iterate, as mgSolve does (poisson.cc:139-149)               */
+    double mg_residual;             /* sr_mg_stats.residual of the pressure solve                  */
 } sr_ts_stats;
 
 /* hydro_d3::hydro_d3 (src/solvers/hydro/hydro_d3.cc:63-119) without the initial condition */

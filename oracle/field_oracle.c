@@ -6,6 +6,7 @@
  *   lib/field/vfield.cc:104-146        computeDiff
  *   lib/field/vfield.cc:163-253        computeNLin (interpolation slices of field.cc:309-575)
  *   lib/field/vfield.cc:299-320        divergence
+ *   lib/field/sfield.cc:86-142         computeDiff, computeNLin of a node field
  *   lib/field/sfield.cc:155-171        gradient (plainsf::gradient is the same expression)
  *   lib/parallel/mpidata.cc:82-306     syncData with faceRanks = self along periodic x / y
  *   lib/boundary/*.cc                  periodicCC/FC, dirichletCC/FC, neumannCC/FC
@@ -226,18 +227,25 @@ void or_ts_deriv(or_ts *t, int which, int order, int dim, double *out) {
     else d2(t, f, F, dim, out);
 }
 
+static const fshape *loc_shape(const or_ts *t, int loc) { return loc == 0 ? &t->sP : &t->sV[loc - 1]; }
+
+/* computeDiff of one field at location loc (0 node, 1..3 the faces of Vx..Vz):
+   H(core) += d2xx; H(core) += d2yy; H(core) += d2zz, each into a zeroed temporary first
+   (vfield.cc:104-146 per component, sfield.cc:86-100) */
+void or_fd_diff(or_ts *t, int loc, const double *F, double *H) {
+    const fshape *f = loc_shape(t, loc);
+    for (int d = 0; d < 3; d++) {
+        memset(t->deriv, 0, sizeof(double) * (size_t)fsize(f));
+        d2(t, f, F, d, t->deriv);
+        for (int i = 0; i < f->n[0]; i++)
+            for (int j = 0; j < f->n[1]; j++)
+                for (int k = 0; k < f->n[2]; k++) H[FI(f, i, j, k)] += t->deriv[FI(f, i, j, k)];
+    }
+}
+
 /* vfield::computeDiff (vfield.cc:104-146): H(core) += d2xx + d2yy + d2zz, component by component */
 void or_ts_compute_diff(or_ts *t) {
-    for (int c = 0; c < 3; c++) {
-        const fshape *f = &t->sV[c];
-        for (int d = 0; d < 3; d++) {
-            memset(t->deriv, 0, sizeof(double) * (size_t)fsize(f));
-            d2(t, f, t->V[c], d, t->deriv);
-            for (int i = 0; i < f->n[0]; i++)
-                for (int j = 0; j < f->n[1]; j++)
-                    for (int k = 0; k < f->n[2]; k++) t->H[c][FI(f, i, j, k)] += t->deriv[FI(f, i, j, k)];
-        }
-    }
+    for (int c = 0; c < 3; c++) or_fd_diff(t, c + 1, t->V[c], t->H[c]);
 }
 
 /* interpolation slices of velocity component d onto the location of component c (field.cc):
@@ -252,30 +260,46 @@ static int slices(int c, int d, int off[4][3]) {
     return 4;
 }
 
-/* vfield::computeNLin (vfield.cc:163-253): H_c(core) -= (sum of slices of V_d) d_d(V_c) / nslices */
-void or_ts_compute_nlin(or_ts *t) {
-    for (int c = 0; c < 3; c++) {
-        const fshape *f = &t->sV[c];
-        for (int d = 0; d < 3; d++) {
-            int off[4][3];
-            int ns = slices(c, d, off);
-            const fshape *fd = &t->sV[d];
-            memset(t->inter, 0, sizeof(double) * (size_t)fsize(f));
-            for (int s = 0; s < ns; s++)
-                for (int i = 0; i < f->n[0]; i++)
-                    for (int j = 0; j < f->n[1]; j++)
-                        for (int k = 0; k < f->n[2]; k++)
-                            t->inter[FI(f, i, j, k)] += t->V[d][FI(fd, i + off[s][0], j + off[s][1], k + off[s][2])];
-            memset(t->deriv, 0, sizeof(double) * (size_t)fsize(f));
-            d1(t, f, t->V[c], d, t->deriv);
+/* computeNLin of the field F at location loc advected by V (vfield.cc:163-253 for a velocity
+   component, sfield.cc:113-142 for a node field): for d = x, y, z in turn,
+   H(core) -= (sum of the interpolation slices of V_d) d_d(F) / nslices.  Slices of V_d onto the
+   location of component c are those of slices(); onto the nodes, the two faces on either side
+   (field.cc:412-434): the core index, then one back along d. */
+void or_fd_nlin(or_ts *t, int loc, const double *F, const double *const *V, double *H) {
+    const fshape *f = loc_shape(t, loc);
+    for (int d = 0; d < 3; d++) {
+        int off[4][3];
+        int ns;
+        if (loc == 0) {
+            memset(off, 0, sizeof(off));
+            off[1][d] = -1;
+            ns = 2;
+        } else {
+            ns = slices(loc - 1, d, off);
+        }
+        const fshape *fd = &t->sV[d];
+        memset(t->inter, 0, sizeof(double) * (size_t)fsize(f));
+        for (int s = 0; s < ns; s++)
             for (int i = 0; i < f->n[0]; i++)
                 for (int j = 0; j < f->n[1]; j++)
-                    for (int k = 0; k < f->n[2]; k++) {
-                        long q = FI(f, i, j, k);
-                        t->H[c][q] -= t->inter[q] * t->deriv[q] / (double)ns;
-                    }
-        }
+                    for (int k = 0; k < f->n[2]; k++)
+                        t->inter[FI(f, i, j, k)] += V[d][FI(fd, i + off[s][0], j + off[s][1], k + off[s][2])];
+        memset(t->deriv, 0, sizeof(double) * (size_t)fsize(f));
+        d1(t, f, F, d, t->deriv);
+        for (int i = 0; i < f->n[0]; i++)
+            for (int j = 0; j < f->n[1]; j++)
+                for (int k = 0; k < f->n[2]; k++) {
+                    long q = FI(f, i, j, k);
+                    H[q] -= t->inter[q] * t->deriv[q] / (double)ns;
+                }
     }
+}
+
+/* vfield::computeNLin (vfield.cc:163-253) of the velocity by itself: H_c(core) -= (sum of slices
+   of V_d) d_d(V_c) / nslices */
+void or_ts_compute_nlin(or_ts *t) {
+    const double *V[3] = {t->V[0], t->V[1], t->V[2]};
+    for (int c = 0; c < 3; c++) or_fd_nlin(t, c + 1, t->V[c], V, t->H[c]);
 }
 
 /* sfield::gradient (sfield.cc:155-171): grad_c on the core of V_c = metric (P(+1) - P) / h */
@@ -310,6 +334,36 @@ void or_ts_divergence(or_ts *t) {
                 double cz = met(t, 2, 1, 1, k) * (t->V[2][FI(&t->sV[2], i, j, k)] - t->V[2][FI(&t->sV[2], i, j, k - 1)]) / t->h[2];
                 t->divV[FI(p, i, j, k)] = a + b + cz;
             }
+}
+
+/* sfield::gradient on caller arrays (sfield.cc:155-171): the core of each component only */
+void or_fd_gradient(or_ts *t, const double *F, double *G0, double *G1, double *G2) {
+    double *G[3] = {G0, G1, G2};
+    gradient(t, F, G);
+}
+
+/* vfield::divergence on caller arrays (vfield.cc:299-320): divV = 0, then its core */
+void or_fd_divergence(or_ts *t, const double *V0, const double *V1, const double *V2, double *div) {
+    const fshape *p = &t->sP;
+    const double *V[3] = {V0, V1, V2};
+    memset(div, 0, sizeof(double) * (size_t)fsize(p));
+    for (int i = 0; i < p->n[0]; i++)
+        for (int j = 0; j < p->n[1]; j++)
+            for (int k = 0; k < p->n[2]; k++) {
+                double a = met(t, 0, 1, 1, i) * (V[0][FI(&t->sV[0], i, j, k)] - V[0][FI(&t->sV[0], i - 1, j, k)]) / t->h[0];
+                double b = met(t, 1, 1, 1, j) * (V[1][FI(&t->sV[1], i, j, k)] - V[1][FI(&t->sV[1], i, j - 1, k)]) / t->h[1];
+                double cz = met(t, 2, 1, 1, k) * (V[2][FI(&t->sV[2], i, j, k)] - V[2][FI(&t->sV[2], i, j, k - 1)]) / t->h[2];
+                div[FI(p, i, j, k)] = a + b + cz;
+            }
+}
+
+/* derivative::calcDerivative1_* / calcDerivative2* of a caller field at location loc into out,
+   zeroed first as every caller zeroes it (vfield.cc:104-253, sfield.cc:86-142) */
+void or_fd_deriv(or_ts *t, int loc, int order, int dim, const double *F, double *out) {
+    const fshape *f = loc_shape(t, loc);
+    memset(out, 0, sizeof(double) * (size_t)fsize(f));
+    if (order == 1) d1(t, f, F, dim, out);
+    else d2(t, f, F, dim, out);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -401,6 +455,37 @@ void or_ts_impose_vbc(or_ts *t, int c) {
         double top = (t->p.probType == 1 && c == 0) ? 1.0 : 0.0;
         wall_bc(f, 0, F, 5, kind, top);
         wall_bc(f, 0, F, 4, kind, 0.0);
+    }
+}
+
+/* Boundary conditions of one field at location loc, as a field's imposeBC runs them
+   (hydro.cc:386-461, scalar.cc:240-255): syncData, the x walls, the y walls, then the top and the
+   bottom wall.  kinds[w] per wall (0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+): 0 periodic, 1 Dirichlet,
+   2 Neumann, 3 left as it is, the CC or FC class chosen by the field's staggering along the wall's direction
+   (lib/boundary/*.cc).  Periodic x and y walls are the syncData exchange; periodic z walls are
+   periodicCC / periodicFC. */
+void or_fd_bc(or_ts *t, int loc, const int *kinds, const double *values, double *F) {
+    const fshape *f = loc_shape(t, loc);
+    int per[3] = {t->p.xPer, t->p.yPer, t->p.zPer};
+    sync_data(t, f, F);
+    for (int d = 0; d < 2; d++) {
+        if (per[d]) continue;
+        for (int w = 2 * d; w <= 2 * d + 1; w++) {
+            if (kinds[w] == 3) continue;
+            int node = f->s[d];
+            int kind = kinds[w] == 1 ? (node ? 2 : 3) : (node ? 4 : 5);
+            wall_bc(f, 0, F, w, kind, values[w]);
+        }
+    }
+    int order[2] = {5, 4};
+    for (int q = 0; q < 2; q++) {
+        int w = order[q];
+        if (kinds[w] == 3) continue;
+        int node = f->s[2];
+        int kind;
+        if (per[2]) kind = node ? 0 : 1;
+        else kind = kinds[w] == 1 ? (node ? 2 : 3) : (node ? 4 : 5);
+        wall_bc(f, per[2], F, w, kind, values[w]);
     }
 }
 
@@ -590,30 +675,10 @@ void or_ts_build_trhs(or_ts *t) {
     const fshape *p = &t->sP;
     long np = fsize(p);
     memset(t->Trhs, 0, sizeof(double) * (size_t)np);
-    for (int d = 0; d < 3; d++) {
-        memset(t->deriv, 0, sizeof(double) * (size_t)np);
-        d2(t, p, t->T, d, t->deriv);
-        for (int i = 0; i < p->n[0]; i++)
-            for (int j = 0; j < p->n[1]; j++)
-                for (int k = 0; k < p->n[2]; k++) t->Trhs[FI(p, i, j, k)] += t->deriv[FI(p, i, j, k)];
-    }
+    or_fd_diff(t, 0, t->T, t->Trhs);
     for (long q = 0; q < np; q++) t->Trhs[q] *= t->kappa;
-    for (int d = 0; d < 3; d++) {
-        const fshape *fd = &t->sV[d];
-        memset(t->deriv, 0, sizeof(double) * (size_t)np);
-        d1(t, p, t->T, d, t->deriv);
-        for (int i = 0; i < p->n[0]; i++)
-            for (int j = 0; j < p->n[1]; j++)
-                for (int k = 0; k < p->n[2]; k++) {
-                    int o[3] = {i, j, k};
-                    o[d] -= 1;
-                    double inter = 0.0;
-                    inter += t->V[d][FI(fd, i, j, k)];
-                    inter += t->V[d][FI(fd, o[0], o[1], o[2])];
-                    long q = FI(p, i, j, k);
-                    t->Trhs[q] -= inter * t->deriv[q] / 2.0;
-                }
-    }
+    const double *V[3] = {t->V[0], t->V[1], t->V[2]};
+    or_fd_nlin(t, 0, t->T, V, t->Trhs);
     for (long q = 0; q < np; q++) {
         t->Trhs[q] *= t->p.dt;
         t->Trhs[q] += t->T[q];

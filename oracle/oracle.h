@@ -155,6 +155,14 @@ int    or_ts_solve_t(or_ts *t);
 void   or_ts_build_trhs(or_ts *t);
 double or_ts_nusselt(const or_ts *t);
 int    or_ts_mg_cycles(const or_ts *t);
+/* field operators on caller arrays of this grid; loc 0 node (P, T), 1..3 the faces of Vx..Vz */
+void   or_fd_deriv(or_ts *t, int loc, int order, int dim, const double *F, double *out);
+void   or_fd_diff(or_ts *t, int loc, const double *F, double *H);
+void   or_fd_nlin(or_ts *t, int loc, const double *F, const double *const *V, double *H);
+void   or_fd_gradient(or_ts *t, const double *F, double *G0, double *G1, double *G2);
+void   or_fd_divergence(or_ts *t, const double *V0, const double *V1, const double *V2, double *div);
+/* kinds per wall (x-, x+, y-, y+, z-, z+): 0 periodic, 1 Dirichlet, 2 Neumann, 3 left as it is */
+void   or_fd_bc(or_ts *t, int loc, const int *kinds, const double *values, double *F);
 
 #ifdef __cplusplus
 }

@@ -252,6 +252,11 @@ class TsParams(C.Structure):
 VX, VY, VZ, PRES, PCORR, MGRHS, HX, HY, HZ, GX, GY, GZ, DIV, TEMP, TRHS = range(15)
 
 
+def _c(a):
+    """float64 C-contiguous view or copy of a read-only input"""
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
 def _ts_bind(L):
     if getattr(L, "_ts_bound", False):
         return
@@ -290,6 +295,12 @@ def _ts_bind(L):
     L.or_ts_solve_t.argtypes = [vp]
     L.or_ts_nusselt.restype = C.c_double
     L.or_ts_nusselt.argtypes = [vp]
+    L.or_fd_deriv.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.or_fd_diff.argtypes = [vp, C.c_int, dp, dp]
+    L.or_fd_nlin.argtypes = [vp, C.c_int, dp, C.POINTER(dp), dp]
+    L.or_fd_gradient.argtypes = [vp, dp, dp, dp, dp]
+    L.or_fd_divergence.argtypes = [vp, dp, dp, dp, dp]
+    L.or_fd_bc.argtypes = [vp, C.c_int, ip, dp, dp]
     L._ts_bound = True
 
 
@@ -406,3 +417,35 @@ class OracleTS:
 
     def nusselt(self):
         return self.L.or_ts_nusselt(self.h)
+
+    # field operators on caller arrays (loc 0 node, 1..3 the faces of Vx..Vz); arrays are float64,
+    # C-contiguous, of the location's padded shape, and are modified in place where written
+    def loc_shape(self, loc):
+        return self.shape(PRES if loc == 0 else loc - 1)
+
+    def fd_deriv(self, loc, order, dim, F):
+        out = np.zeros(self.loc_shape(loc))
+        self.L.or_fd_deriv(self.h, loc, order, dim, _ptr(_c(F)), _ptr(out))
+        return out
+
+    def fd_diff(self, loc, F, H):
+        self.L.or_fd_diff(self.h, loc, _ptr(_c(F)), _ptr(H))
+
+    def fd_nlin(self, loc, F, V, H):
+        Vc = [_c(v) for v in V]
+        arr = (C.POINTER(C.c_double) * 3)(*[_ptr(v) for v in Vc])
+        self.L.or_fd_nlin(self.h, loc, _ptr(_c(F)), arr, _ptr(H))
+
+    def fd_gradient(self, F, G):
+        self.L.or_fd_gradient(self.h, _ptr(_c(F)), *[_ptr(g) for g in G])
+
+    def fd_divergence(self, V):
+        div = np.zeros(self.shape(PRES))
+        Vc = [_c(v) for v in V]
+        self.L.or_fd_divergence(self.h, *[_ptr(v) for v in Vc], _ptr(div))
+        return div
+
+    def fd_bc(self, loc, kinds, values, F):
+        k = (C.c_int * 6)(*kinds)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self.L.or_fd_bc(self.h, loc, k, _ptr(v), _ptr(F))

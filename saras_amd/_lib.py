@@ -1,4 +1,4 @@
-"""ctypes binding of the C ABI in include/saras_mg.h and include/saras_ts.h (libsaras_mg.so, gfx950).
+"""ctypes binding of the C ABI in include/*.h (libsaras_mg.so, gfx950).
 
 The library is loaded from saras_amd/_build; a missing library is an error (there is no CPU
 fallback for any solver entry point)."""
@@ -65,7 +65,8 @@ class SrTsDesc(C.Structure):
 class SrTsStats(C.Structure):
     _fields_ = [("jacobi_iters", C.c_int * 4), ("mg_cycles", C.c_int), ("step_ms", C.c_double),
                 ("solve_ms", C.c_double), ("guess_ms", C.c_double), ("guess_choice", C.c_int),
-                ("rel_guess", C.c_double), ("train_loss", C.c_double), ("sweep0_ms", C.c_double)]
+                ("rel_guess", C.c_double), ("train_loss", C.c_double), ("sweep0_ms", C.c_double),
+                ("mg_converged", C.c_int), ("mg_residual", C.c_double)]
 
 
 SR_GUESS_LAYERS, SR_GUESS_IN_CH = 12, 8
@@ -80,6 +81,27 @@ class SrGuessDesc(C.Structure):
 class SrGuessStats(C.Structure):
     _fields_ = [("rel_warm", C.c_double), ("rel_extrap", C.c_double), ("rel_learned", C.c_double),
                 ("choice", C.c_int), ("predict_ms", C.c_double), ("rel_linear", C.c_double), ("net_used", C.c_int)]
+
+
+SR_LOC_NODE, SR_LOC_VX, SR_LOC_VY, SR_LOC_VZ = 0, 1, 2, 3
+SR_BC_PERIODIC, SR_BC_DIRICHLET, SR_BC_NEUMANN, SR_BC_NONE = 0, 1, 2, 3
+
+
+class SrFgridDesc(C.Structure):
+    _fields_ = [("xInd", C.c_int), ("yInd", C.c_int), ("zInd", C.c_int),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("xGrid", C.c_int), ("yGrid", C.c_int), ("zGrid", C.c_int),
+                ("betaX", C.c_double), ("betaY", C.c_double), ("betaZ", C.c_double),
+                ("xPer", C.c_int), ("yPer", C.c_int), ("zPer", C.c_int), ("iScheme", C.c_int)]
+
+
+class SrBc(C.Structure):
+    _fields_ = [("kind", C.c_int * 6), ("value", C.c_double * 6)]
+
+
+class SrJacobiDesc(C.Structure):
+    _fields_ = [("form", C.c_int), ("dt", C.c_double), ("Re", C.c_double), ("coef", C.c_double),
+                ("cnTolerance", C.c_double), ("zCollocMetric", C.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/*.h
@@ -153,6 +175,21 @@ SIGNATURES = [
     ("sr_guess_kappa", C.c_int, [_vp, _dp, _ip]),
     ("sr_guess_extrapolate", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sr_guess_set_policy", C.c_int, [_vp, C.c_int]),
+    ("sr_fgrid_create", C.c_int, [C.POINTER(SrFgridDesc), C.POINTER(C.c_void_p)]),
+    ("sr_fgrid_destroy", C.c_int, [_vp]),
+    ("sr_fgrid_dims", C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
+    ("sr_fgrid_stream", C.c_int, [_vp, C.POINTER(C.c_void_p)]),
+    ("sr_fgrid_sync", C.c_int, [_vp]),
+    ("sr_deriv1", C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
+    ("sr_deriv2", C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
+    ("sr_vf_diff", C.c_int, [_vp] + [_vp] * 6),
+    ("sr_sf_diff", C.c_int, [_vp, _vp, _vp]),
+    ("sr_vf_nlin", C.c_int, [_vp] + [_vp] * 9),
+    ("sr_sf_nlin", C.c_int, [_vp] + [_vp] * 5),
+    ("sr_divergence", C.c_int, [_vp] + [_vp] * 4),
+    ("sr_gradient", C.c_int, [_vp] + [_vp] * 4),
+    ("sr_bc_fill", C.c_int, [_vp, C.c_int, C.POINTER(SrBc), _vp]),
+    ("sr_jacobi_visc", C.c_int, [_vp, C.c_int, C.POINTER(SrJacobiDesc), C.POINTER(SrBc), _vp, _vp, _vp, _ip]),
 ]
 
 _lib = None

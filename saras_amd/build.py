@@ -19,7 +19,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unknown-warning-option",
 ]
-HIP_SOURCES = ["mg_kernels.hip", "mg_api.hip", "mg_solve.hip", "ts_kernels.hip", "ts_api.hip", "unet_kernels.hip", "unet_api.hip", "guess_solver.hip"]
+HIP_SOURCES = ["mg_kernels.hip", "mg_api.hip", "mg_solve.hip", "ts_kernels.hip", "ts_api.hip", "unet_kernels.hip", "unet_api.hip", "guess_solver.hip", "field_api.hip"]
 CPP_SOURCES = ["grid.cpp"]
 
 

@@ -14,6 +14,7 @@
 #include "../../include/saras_ts.h"
 #include "mg_state.h"
 #include "ts_kernels.h"
+#include "ghost_waves.h"
 
 using namespace sr;
 
@@ -67,91 +68,6 @@ static long fsize(const FShape &f) { return (long)(f.n[0] + 2) * (f.n[1] + 2) * 
 // ------------------------------------------------------------------------------------------------
 // ghost waves
 // ------------------------------------------------------------------------------------------------
-static void add_op(GhostWave &w, const GhostOp &op) {
-    if (op.count > 0) w.ops[w.nops++] = op;
-}
-
-// mpidata::syncData (mpidata.cc:278-306) when the neighbour along periodic x / y is the rank itself:
-// the faces over the core of the other two directions, then the four x-y edges over the z core.
-// The edge exchange runs whatever the periodicity (edge ranks are never cleared, hydro.cc:174-212).
-static GhostWave sync_wave(const FShape &f, int xPer, int yPer) {
-    GhostWave w{};
-    int nx = f.n[0], ny = f.n[1], nz = f.n[2], sx = f.s[0], sy = f.s[1];
-    auto plane = [&](int dim, int dst, int src, int alo, int ahi, int blo, int bhi) {
-        GhostOp o{};
-        o.kind = 0;
-        o.dim = dim;
-        o.dst[0] = dst;
-        o.src[0] = src;
-        o.lo[0] = alo; o.hi[0] = ahi;
-        o.lo[1] = blo; o.hi[1] = bhi;
-        o.a = 1.0; o.b = 0.0;
-        o.count = (long)(ahi - alo + 1) * (bhi - blo + 1);
-        add_op(w, o);
-    };
-    auto line = [&](int dx, int dy, int sxv, int syv) {
-        GhostOp o{};
-        o.kind = 1;
-        o.dim = 2;
-        o.dst[0] = dx; o.dst[1] = dy;
-        o.src[0] = sxv; o.src[1] = syv;
-        o.lo[0] = 0; o.hi[0] = nz - 1;
-        o.a = 1.0; o.b = 0.0;
-        o.count = nz;
-        add_op(w, o);
-    };
-    if (xPer) {
-        plane(0, -1, nx - 1 - sx, 0, ny - 1, 0, nz - 1);
-        plane(0, nx, sx, 0, ny - 1, 0, nz - 1);
-    }
-    if (yPer) {
-        plane(1, -1, ny - 1 - sy, 0, nx - 1, 0, nz - 1);
-        plane(1, ny, sy, 0, nx - 1, 0, nz - 1);
-    }
-    line(-1, -1, nx - 1 - sx, ny - 1 - sy);
-    line(-1, ny, nx - 1 - sx, sy);
-    line(nx, -1, sx, ny - 1 - sy);
-    line(nx, ny, sx, sy);
-    return w;
-}
-
-// one wall of a field (lib/boundary/*.cc): the wall slice is the layer just outside the bulk
-// (field.cc:180-290).  kind 0 periodicCC, 1 periodicFC, 2 dirichletCC, 3 dirichletFC,
-// 4 neumannCC, 5 neumannFC.
-static GhostOp wall_op(const FShape &f, int per, int wall, int kind, double value) {
-    int d = wall / 2, hi = wall & 1;
-    int n = f.n[d];
-    int blo = 0, bhi = n - 1;
-    if (f.s[d] && !per) {
-        blo = 1;
-        bhi = n - 2;
-    }
-    int w = hi ? bhi + 1 : blo - 1;
-    int wo = hi ? blo - 1 : bhi + 1;
-    int src;
-    switch (kind) {
-        case 0: src = hi ? wo + 2 : wo - 2; break;
-        case 1: src = hi ? wo + 1 : wo - 1; break;
-        case 3: case 4: case 5: src = hi ? w - 1 : w + 1; break;
-        default: src = w; break;
-    }
-    GhostOp o{};
-    o.kind = 0;
-    o.dim = d;
-    o.dst[0] = w;
-    o.src[0] = src;
-    int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
-    o.lo[0] = -1; o.hi[0] = f.n[a];
-    o.lo[1] = -1; o.hi[1] = f.n[b];
-    o.count = (long)(f.n[a] + 2) * (f.n[b] + 2);
-    switch (kind) {
-        case 2: o.a = 0.0; o.b = value; break;
-        case 3: o.a = -1.0; o.b = 2.0 * value; break;     // 2 v - F(wall +- 1)
-        default: o.a = 1.0; o.b = 0.0; break;
-    }
-    return o;
-}
-
 // hydro::imposeU/V/WBCs (hydro.cc:386-461) with the walls of hydro::initVBC (hydro.cc:287-371):
 // syncData, then x walls, y walls, and the top and bottom walls.  Walls normal to the component are
 // no-penetration (dirichletFC 0), the others no-slip (dirichletCC 0; the lid of probType 1 moves
@@ -645,7 +561,10 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     }
     sr_mg_stats ms;
     auto ts0 = std::chrono::steady_clock::now();
-    RETCHK(sr_mg_solve(t->mg, t->Pp, t->mgRHS, 1, &ms));
+    // mgSolve never fails: after vcCount cycles it returns its last iterate and the step goes on
+    // (poisson.cc:139-151); a solve that misses its stop rule is reported in the statistics
+    int srs = sr_mg_solve(t->mg, t->Pp, t->mgRHS, 1, &ms);
+    if (srs != SR_OK && srs != SR_ERR_NOT_CONVERGED) return srs;
     auto ts1 = std::chrono::steady_clock::now();
     t->mg_cycles = ms.cycles;
     // the solver's own evaluation of the guess, where it makes one, is exact (the predictor's fit
@@ -692,6 +611,8 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
         st->rel_guess = rel_guess;
         st->train_loss = loss;
         st->mg_cycles = t->mg_cycles;
+        st->mg_converged = ms.converged;
+        st->mg_residual = ms.residual;
         st->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
     return SR_OK;

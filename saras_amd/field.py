@@ -1,0 +1,107 @@
+"""Python mirror of the field operators of lib/field over the C ABI in include/saras_field.h.
+
+The names follow the reference: a `FieldGrid` stands for grid::grid plus the constant factors every
+`derivative` holds (derivative.cc:60-105); `calcDerivative1_x` ... `calcDerivative2zz`,
+`computeDiff`, `computeNLin`, `divergence`, `gradient`, `imposeBC` and the Jacobi solves of
+hydro_d3 / scalar_d3 take fields that live in HBM (`saras_amd._lib.DeviceArray`, padded layout,
+k fastest).  There is no CPU path: a missing library raises."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as _L
+
+NODE, VX, VY, VZ = _L.SR_LOC_NODE, _L.SR_LOC_VX, _L.SR_LOC_VY, _L.SR_LOC_VZ
+PERIODIC, DIRICHLET, NEUMANN, NONE = _L.SR_BC_PERIODIC, _L.SR_BC_DIRICHLET, _L.SR_BC_NEUMANN, _L.SR_BC_NONE
+
+
+def fgrid_desc(ind=(5, 5, 5), L=(1.0, 1.0, 1.0), mesh="UUU", beta=(1.0, 1.0, 1.0), domain="PPP", i_scheme=1):
+    d = _L.SrFgridDesc()
+    d.xInd, d.yInd, d.zInd = ind
+    d.Lx, d.Ly, d.Lz = L
+    d.xGrid, d.yGrid, d.zGrid = (1 if c == "U" else 2 for c in mesh)
+    d.betaX, d.betaY, d.betaZ = beta
+    d.xPer, d.yPer, d.zPer = (int(c == "P") for c in domain)
+    d.iScheme = i_scheme
+    return d
+
+
+def bc(kinds, values=(0.0,) * 6):
+    b = _L.SrBc()
+    for w in range(6):
+        b.kind[w] = kinds[w]
+        b.value[w] = values[w]
+    return b
+
+
+class FieldGrid:
+    def __init__(self, desc):
+        self.L = _L.load()
+        self.desc = desc
+        h = C.c_void_p()
+        _L.check(self.L.sr_fgrid_create(C.byref(desc), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            self.L.sr_fgrid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, loc):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _L.check(self.L.sr_fgrid_dims(self.h, loc, C.byref(a), C.byref(b), C.byref(c)))
+        return (a.value + 2, b.value + 2, c.value + 2)
+
+    def array(self, loc, host=None):
+        """A device array of a location's padded shape, optionally filled from `host`."""
+        d = _L.DeviceArray(self.shape(loc))
+        d.upload(np.zeros(d.shape) if host is None else host)
+        return d
+
+    def sync(self):
+        _L.check(self.L.sr_fgrid_sync(self.h))
+
+    # derivative (derivative.h:78-84)
+    def calcDerivative1(self, loc, dim, F, out):
+        _L.check(self.L.sr_deriv1(self.h, loc, dim, F.ptr, out.ptr))
+
+    def calcDerivative2(self, loc, dim, F, out):
+        _L.check(self.L.sr_deriv2(self.h, loc, dim, F.ptr, out.ptr))
+
+    # vfield (vfield.h:76-80) and sfield (sfield.h:73-76)
+    def vfComputeDiff(self, V, H):
+        _L.check(self.L.sr_vf_diff(self.h, *(v.ptr for v in V), *(x.ptr for x in H)))
+
+    def sfComputeDiff(self, F, H):
+        _L.check(self.L.sr_sf_diff(self.h, F.ptr, H.ptr))
+
+    def vfComputeNLin(self, U, V, H):
+        _L.check(self.L.sr_vf_nlin(self.h, *(u.ptr for u in U), *(v.ptr for v in V), *(x.ptr for x in H)))
+
+    def sfComputeNLin(self, V, F, H):
+        _L.check(self.L.sr_sf_nlin(self.h, *(v.ptr for v in V), F.ptr, H.ptr))
+
+    def divergence(self, V, div):
+        _L.check(self.L.sr_divergence(self.h, *(v.ptr for v in V), div.ptr))
+
+    def gradient(self, F, G):
+        _L.check(self.L.sr_gradient(self.h, F.ptr, *(g.ptr for g in G)))
+
+    # boundary classes + syncData
+    def imposeBC(self, loc, bcs, F):
+        _L.check(self.L.sr_bc_fill(self.h, loc, C.byref(bcs), F.ptr))
+
+    def jacobi(self, loc, form, dt, bcs, R, F, G, Re=1.0, coef=0.0, cn_tol=1e-6, z_colloc=False):
+        """hydro_d3::solveV* (form 0, with Re) or scalar_d3's solves (form 1, with coef); returns
+        the number of sweeps."""
+        j = _L.SrJacobiDesc()
+        j.form, j.dt, j.Re, j.coef, j.cnTolerance, j.zCollocMetric = form, dt, Re, coef, cn_tol, int(z_colloc)
+        it = C.c_int()
+        _L.check(self.L.sr_jacobi_visc(self.h, loc, C.byref(j), C.byref(bcs), R.ptr, F.ptr, G.ptr, C.byref(it)))
+        return it.value

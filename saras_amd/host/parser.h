@@ -16,6 +16,7 @@ class parser {
     real Lx = 1.0, Ly = 1.0, Lz = 1.0;
     real betaX = 1.0, betaY = 1.0, betaZ = 1.0;
     real mgTolerance = 1.0e-6;
+    int iScheme = 1;
 
     // additions of this implementation (no reference counterpart)
     bool testPoisson = false;    // the TEST_POISSON compile flag of the reference, as a run-time switch

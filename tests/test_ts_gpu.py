@@ -192,3 +192,29 @@ def test_time_steps(case):
         assert abs(e_o - e_g) <= RTOL * abs(e_o), (step, e_o, e_g)
     g.close()
     o.close()
+
+
+def test_step_completes_when_pressure_solve_misses_its_tolerance():
+    """mgSolve never fails (poisson.cc:139-151): with one V-cycle allowed and an unreachable
+    relative tolerance the step still projects with the last iterate, like the oracle's step, and
+    reports the solve as unconverged."""
+    from saras_amd import ts, _lib
+    kw = dict(CASES["ppn_tanh_walls"], vc_count=1, res_type=2)
+    o = po.OracleTS(po.ts_params(semantics=po.SEM_PROJECTED, rel_tol=1e-30, **kw))
+    g = ts.Hydro(ts.ts_desc(semantics=_lib.SR_SEM_PROJECTED, rel_tol=1e-30, **kw))
+    o.init_tgv()
+    g.initTGV()
+    for step in range(2):
+        o.time_advance()
+        st = g.timeAdvance()
+        assert st.mg_cycles == 1 == o.mg_cycles()
+        assert st.mg_converged == 0 and 0.0 < st.mg_residual < 1.0, (st.mg_converged, st.mg_residual)
+        vs = _velocity_scale(o)
+        for c in range(3):
+            _assert_field(g, o, c, c, what=f"step {step} V{c}", scale=vs)
+        a, b = g.download(ts.P), o.array(po.PRES)
+        a, b = a - a[1:-1, 1:-1, 1:-1].mean(), b - b[1:-1, 1:-1, 1:-1].mean()
+        ok, err, s = rel_close(a, b, RTOL, scale=float(np.max(np.abs(o.array(po.PRES)))))
+        assert ok, (f"step {step} P", err, s)
+    g.close()
+    o.close()
