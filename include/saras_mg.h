@@ -27,6 +27,7 @@ extern "C" {
 #define SR_ERR_STATE         3   /* operation not valid in the current state (e.g. level)     */
 #define SR_ERR_NOMEM         4
 #define SR_ERR_NOT_CONVERGED 5   /* solve hit vcCount without meeting its stop rule           */
+#define SR_ERR_IO            6   /* file error (restart files, saras_restart.h)               */
 
 /* which level array (lib/poisson/poisson.h:70-73) */
 #define SR_PRESSURE 0            /* pressureData */

@@ -86,6 +86,9 @@ int sr_ts_init_tgv(sr_ts *ts);
 int sr_ts_upload(sr_ts *ts, int which, const double *host);
 int sr_ts_download(sr_ts *ts, int which, double *host);
 int sr_ts_field_dims(const sr_ts *ts, int which, int *nx, int *ny, int *nz);   /* core sizes */
+/* the device array of a field, in the padded layout (for probes, sr_probe_gather, and for the field
+   operators of saras_field.h on the handle's fields); valid until sr_ts_destroy */
+int sr_ts_field_ptr(sr_ts *ts, int which, double **dev);
 
 /* hydro_d3::timeAdvance (hydro_d3.cc:254-368), zero forcing; with desc.scalar the step of
  * scalar_d3::timeAdvance (scalar_d3.cc:243-370) with buoyancy                                   */

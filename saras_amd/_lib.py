@@ -8,7 +8,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libsaras_mg.so")
 
-SR_OK, SR_ERR_ARG, SR_ERR_HIP, SR_ERR_STATE, SR_ERR_NOMEM, SR_ERR_NOT_CONVERGED = 0, 1, 2, 3, 4, 5
+SR_OK, SR_ERR_ARG, SR_ERR_HIP, SR_ERR_STATE, SR_ERR_NOMEM, SR_ERR_NOT_CONVERGED, SR_ERR_IO = 0, 1, 2, 3, 4, 5, 6
 SR_PRESSURE, SR_RESIDUAL, SR_TMP = 0, 1, 2
 SR_SEM_REFERENCE, SR_SEM_PROJECTED = 0, 1
 SR_PREC_FP64, SR_PREC_MIXED, SR_PREC_MIXED_GCR = 0, 1, 2
@@ -190,6 +190,8 @@ SIGNATURES = [
     ("sr_gradient", C.c_int, [_vp] + [_vp] * 4),
     ("sr_bc_fill", C.c_int, [_vp, C.c_int, C.POINTER(SrBc), _vp]),
     ("sr_jacobi_visc", C.c_int, [_vp, C.c_int, C.POINTER(SrJacobiDesc), C.POINTER(SrBc), _vp, _vp, _vp, _ip]),
+    ("sr_probe_gather", C.c_int, [C.c_int, C.POINTER(C.c_void_p), _ip, C.c_int, _ip, _dp]),
+    ("sr_ts_field_ptr", C.c_int, [_vp, C.c_int, C.POINTER(C.c_void_p)]),
 ]
 
 _lib = None

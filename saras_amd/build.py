@@ -19,7 +19,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unknown-warning-option",
 ]
-HIP_SOURCES = ["mg_kernels.hip", "mg_api.hip", "mg_solve.hip", "ts_kernels.hip", "ts_api.hip", "unet_kernels.hip", "unet_api.hip", "guess_solver.hip", "field_api.hip"]
+HIP_SOURCES = ["mg_kernels.hip", "mg_api.hip", "mg_solve.hip", "ts_kernels.hip", "ts_api.hip", "unet_kernels.hip", "unet_api.hip", "guess_solver.hip", "field_api.hip", "probes.hip"]
 CPP_SOURCES = ["grid.cpp"]
 
 
@@ -61,6 +61,23 @@ def build_host():
     return exe
 
 
+IO_LIB = os.path.join(OUT, "libsaras_io.so")
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
+
+
+def build_io():
+    """Restart files (include/saras_restart.h): host C++ over the image's serial HDF5."""
+    src = os.path.join(PKG, "io", "restart.cc")
+    if not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
+        raise RuntimeError(f"hdf5.h not found under {HDF5_ROOT}/include (set HDF5_ROOT)")
+    os.makedirs(OUT, exist_ok=True)
+    tmp = IO_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(HDF5_ROOT, "include"), src,
+          "-L", os.path.join(HDF5_ROOT, "lib"), "-lhdf5", f"-Wl,-rpath,{os.path.join(HDF5_ROOT, 'lib')}", "-o", tmp])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 def build_oracle():
     subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], check=True)
 
@@ -68,6 +85,7 @@ def build_oracle():
 def build_all():
     build_lib()
     build_host()
+    build_io()
     build_oracle()
 
 

@@ -340,6 +340,12 @@ extern "C" int sr_ts_field_dims(const sr_ts *t, int which, int *nx, int *ny, int
     return SR_OK;
 }
 
+extern "C" int sr_ts_field_ptr(sr_ts *t, int which, double **dev) {
+    if (!t || !dev) return set_err(SR_ERR_ARG, "null argument");
+    const FShape *f;
+    return field(t, which, dev, &f);
+}
+
 extern "C" int sr_ts_upload(sr_ts *t, int which, const double *host) {
     if (!t || !host) return set_err(SR_ERR_ARG, "null argument");
     double *p;

@@ -5,13 +5,15 @@
   (parser.cc:249-340), raising ValueError where the reference prints and exits.
 * ``desc_from``: the run descriptor (include/saras_ts.h) of a parameter set.  Options this build does
   not implement raise NotImplementedError instead of running something else (adaptive time step,
-  restart, channel flows and their forcings, rotation, the heating plate, single-sided stretching,
+  channel flows and their forcings, rotation, the heating plate, single-sided stretching,
   fourth-order differences, Euler integration).
 * ``TimeSeries``: output/TimeSeries.dat and the console line of tseries (lib/io/tseries.cc:62-246),
   the energy, divergence and Nusselt number reduced on the device.
 * ``solve_pde``: the time loop of hydro_d3::solvePDE (src/solvers/hydro/hydro_d3.cc:131-240) and its
-  scalar_d3 counterpart, writing the time series every ``ioCnt`` steps until ``tMax``.  Solution,
-  restart and probe files (HDF5) are not written.
+  scalar_d3 counterpart: the time series every ``ioCnt`` steps, probes every ``prInt``
+  (output/ProbesData.dat, saras_amd.probes) and the restart file every ``rsInt``
+  (output/restartFile.h5, saras_amd.restart) until ``tMax``; with ``restartFlag`` the run starts
+  from the restart file.  Solution files (writeSolution / writeTarang) are not written.
 
 The MPI decomposition keys are read and validated as the reference does, but one GPU holds the
 whole domain.
@@ -156,8 +158,6 @@ def desc_from(p, **overrides):
     unsupported = []
     if p.useCFL:
         unsupported.append("Use CFL Condition (adaptive time step)")
-    if p.restartFlag:
-        unsupported.append("Restart Run (HDF5 restart files)")
     if p.dScheme != 1:
         unsupported.append("Differentiation Scheme other than 1")
     if p.iScheme != 1:
@@ -257,26 +257,54 @@ def initialise(h, p, seed=0, amp=0.0):
 
 
 def solve_pde(p, out_dir=".", log=print, seed=0, amp=0.0, max_steps=None, **overrides):
-    """The time loop of solvePDE: the time series at t = 0, then timeAdvance, t += dt, a time-series
-    line every ioCnt steps, until |tMax - t| < dt / 2 (hydro_d3.cc:165-235).  Returns the TimeSeries
-    rows and the number of steps; max_steps stops early (for tests)."""
+    """The time loop of solvePDE (hydro_d3.cc:131-235): the initial state (or the restart file's),
+    the time series, probes and, for fresh runs, nothing else at t = 0; then timeAdvance, t += dt, a
+    time-series line every ioCnt steps, probes when |prTime - t| < dt / 2, the restart file when
+    |rsTime - t| < dt / 2, until |tMax - t| < dt / 2.  The next probe and restart times start at
+    t + interval - fmod(t, interval), as the reference sets them.  Returns the TimeSeries rows and the
+    number of steps; max_steps stops early (for tests)."""
+    from . import probes as probes_mod
+    from . import restart
     h = ts.Hydro(desc_from(p, **overrides))
+    series = prb = None
     try:
-        initialise(h, p, seed=seed, amp=amp)
-        series = TimeSeries(h, p, os.path.join(out_dir, "output", "TimeSeries.dat"), log=log)
-        time, dt, steps = 0.0, p.tStp, 0
+        out = os.path.join(out_dir, "output")
+        rs_path = os.path.join(out, "restartFile.h5")
+        if p.restartFlag:
+            time = restart.load(h, rs_path)
+        else:
+            initialise(h, p, seed=seed, amp=amp)
+            time = 0.0
+        series = TimeSeries(h, p, os.path.join(out, "TimeSeries.dat"), log=log)
+        if p.readProbes:
+            plist = probes_mod.check_probes(probes_mod.parse_probes(p.probeCoords), (p.xInd, p.yInd, p.zInd))
+            prb = probes_mod.Probes(h, plist, os.path.join(out, "ProbesData.dat"), restart=p.restartFlag)
+        dt, steps = p.tStp, 0
         series.write(time, dt)
+        if not p.restartFlag and prb is not None:
+            prb.probe_data(time)
+        pr_time = time + p.prInt - math.fmod(time, p.prInt)
+        rs_time = time + p.rsInt - math.fmod(time, p.rsInt)
         while True:
             h.timeAdvance()
             steps += 1
             time += dt
             if steps % p.ioCnt == 0:
                 series.write(time, dt)
+            if prb is not None and abs(pr_time - time) < 0.5 * dt:
+                prb.probe_data(time)
+                pr_time += p.prInt
+            if abs(rs_time - time) < 0.5 * dt:
+                restart.save(h, rs_path, time)
+                rs_time += p.rsInt
             if abs(p.tMax - time) < 0.5 * dt or (max_steps is not None and steps >= max_steps):
                 break
-        series.close()
         return series.rows, steps
     finally:
+        if series is not None:
+            series.close()
+        if prb is not None:
+            prb.close()
         h.close()
 
 

@@ -67,6 +67,12 @@ class Hydro:
             pass
 
     # -- fields -----------------------------------------------------------------------------
+    def device_ptr(self, which):
+        """Device address of a field's padded array (valid while the handle lives)."""
+        p = C.c_void_p()
+        _L.check(self.L.sr_ts_field_ptr(self.h, which, C.byref(p)))
+        return p.value
+
     def shape(self, which):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _L.check(self.L.sr_ts_field_dims(self.h, which, C.byref(a), C.byref(b), C.byref(c)))

@@ -160,7 +160,6 @@ def test_descriptor_of_a_parameter_set():
 
 @pytest.mark.parametrize("changes,what", [
     ({"Use CFL Condition": "true"}, "CFL"),
-    (dict(Restart_Run="true"), "Restart"),
     (dict(Mesh_Type='"USU"'), "single-sided"),
     (dict(Problem_Type="3"), "Problem Type 3"),
     (dict(Force="2"), "Force 2"),
