@@ -15,6 +15,8 @@ V(2,2) red-black Gauss-Seidel cycles with vcDepth 7, stop on ||f - L u||_2 / ||f
   tgv129      one step = one Taylor-Green hydro_d3 time step at 129^3
   mixed257x16 the 257^3 case for seeds 0..15: FP64, mixed refinement and GCR mixed solves, the
               mixed solutions' errors against the FP64 solve, seed 0 against the oracle
+  unet129     the learned guess's network on 129^3: each layer's MFMA convolution, adjoint and weight
+              gradient timed, TFLOP/s against the dense BF16 peak
 
 Inputs are resident in HBM before the timed region.  Prints one JSON line.  ``--gpus`` above 1 is
 refused: this build runs on one GPU.
@@ -198,6 +200,15 @@ def tgv_bench(args):
         ke.append(h.energy())
     t1 = time.perf_counter()
     maxdiv = h.maxDivergence()
+    # the step kernels one by one (HIP events on the handle's stream) against their algorithmic
+    # bytes; timed after the stepped state was read, since they overwrite what they write
+    kernels = {}
+    for kid, name in ((0, "k_build_rhs x3"), (1, "k_jacobi_v (Vx)"), (2, "k_jacobi_res (Vx)"),
+                      (3, "k_divergence"), (4, "k_project x3"), (5, "k_energy")):
+        kms = h.time_kernel(kid, reps=20)
+        kb = h.kernel_bytes(kid)
+        gbs = kb / (kms * 1e-3) / 1e9
+        kernels[name] = {"ms": round(kms, 5), "bytes": kb, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     h.close()
     ms = (t1 - t0) * 1e3 / args.steps
     cpu = None
@@ -239,7 +250,13 @@ def tgv_bench(args):
                    "median_step_ms": float(np.median(step_ms)), "jacobi_sweeps_per_step": jac[-1],
                    "mg_cycles_median": int(np.median(cyc)), "ke_first": ke[0], "ke_last": ke[-1],
                    "ke_trace_every_10": ke[::10], "max_divergence_end": maxdiv,
-                   "ke_rel_diff_vs_oracle_first_steps": ke_check, "ke_steps_checked_vs_oracle": ke_steps_checked},
+                   "ke_rel_diff_vs_oracle_first_steps": ke_check, "ke_steps_checked_vs_oracle": ke_steps_checked,
+                   "step_kernels": kernels},
+        "roofline": {"bound": "hbm", "kernel": "k_build_rhs x3 (the largest non-multigrid kernel of the step)",
+                     "achieved": kernels["k_build_rhs x3"]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kernels["k_build_rhs x3"]["frac"], "traffic": None,
+                     "bytes_per_launch": kernels["k_build_rhs x3"]["bytes"],
+                     "avg_launch_ms": kernels["k_build_rhs x3"]["ms"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -493,6 +510,64 @@ def seq_bench(args):
     print(json.dumps(out))
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak (no sparsity)
+
+
+def unet_bench(args):
+    """The learned guess's network on its 129^3 grid (the network grid of the 257^3 solves): one
+    forward and one backward pass through the host entry points to fill every tensor, then each of
+    the 12 layers' convolution, adjoint convolution and weight gradient timed over --steps launches
+    with HIP events on the network's stream.  FLOPs per launch = 2 * 27 * cin * cout * voxels.  The
+    line's value is the time of one forward plus one backward pass of the convolutions; the roofline
+    is priced on the dominant kernel against the dense BF16 MFMA peak."""
+    import numpy as np
+    from saras_amd import guess
+    n = (129, 129, 129)
+    net = guess.GuessNet(n, fine=(257, 257, 257), seed=0, history=0)
+    rng = np.random.default_rng(0)
+    net.forward(rng.standard_normal(n + (guess.IN_CH,)).astype(np.float32))
+    net.backward(rng.standard_normal(n).astype(np.float32) * 1e-2)
+    layers, fwd_ms, bwd_ms, fwd_fl, bwd_fl = [], 0.0, 0.0, 0.0, 0.0
+    for l in range(12):
+        ci, co, lv = guess.layer_info(l)
+        rec = {"layer": guess.LAYER_NAMES[l], "cin": ci, "cout": co, "level": lv}
+        for which, key in ((0, "conv"), (1, "adjoint"), (2, "wgrad")):
+            if which == 1 and l == 0:
+                continue   # the input layer needs no data gradient
+            ms, fl = net.time_layer(l, which, reps=max(args.steps, 3))
+            rec[key + "_ms"] = round(ms, 5)
+            rec[key + "_TFLOPs"] = round(fl / (ms * 1e-3) / 1e12, 2)
+            if which == 0:
+                fwd_ms += ms
+                fwd_fl += fl
+            else:
+                bwd_ms += ms
+                bwd_fl += fl
+        rec["GFLOP"] = round(fl / 1e9, 3)
+        layers.append(rec)
+    net.close()
+    # dominant kernel: the largest total time over the three kinds
+    dom = max(((r[k + "_ms"], r["layer"], k, r[k + "_TFLOPs"]) for r in layers for k in ("conv", "adjoint", "wgrad")
+               if k + "_ms" in r))
+    total_ms = fwd_ms + bwd_ms
+    res = {
+        "metric": "ms per forward + backward pass of the guess network's convolutions, 129^3 grid",
+        "value": round(total_ms, 4), "unit": "ms/pass", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+        "ms_per_step": round(total_ms, 4), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": "3D U-Net of the learned guess (16/32/64/64 channels, 12 3x3x3 convolutions) on "
+                               "129^3 nodes: every layer's convolution, adjoint and weight gradient",
+                   "forward_ms": round(fwd_ms, 4), "forward_TFLOPs": round(fwd_fl / (fwd_ms * 1e-3) / 1e12, 2),
+                   "backward_ms": round(bwd_ms, 4), "backward_TFLOPs": round(bwd_fl / (bwd_ms * 1e-3) / 1e12, 2),
+                   "layers": layers},
+        "roofline": {"bound": "mfma", "kernel": f"{dom[2]} of layer {dom[1]}", "achieved": dom[3],
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(dom[3] / MFMA_BF16_PEAK_TFLOPS, 4),
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,7 +583,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="tgv129: time steps of the oracle's red-black step compared with the device's energy trace")
     ap.add_argument("--workload", default="rbc257",
-                    choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129", "mixed257x16"])
+                    choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129", "mixed257x16", "unet129"])
     ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
     ap.add_argument("--pad-mode", type=int, default=1, choices=[0, 1],
                     help="convection workloads: residual pads of the mixed solves' restriction (0 the reference's "
@@ -530,6 +605,9 @@ def main():
         return
     if args.workload == "tgv129":
         tgv_bench(args)
+        return
+    if args.workload == "unet129":
+        unet_bench(args)
         return
     if args.workload.startswith("rbc"):
         rbc_bench(args)

@@ -103,6 +103,12 @@ int sr_guess_extrapolate(sr_guess *g, const double *p1, const double *p2, double
 int sr_guess_set_policy(sr_guess *g, int policy);
 /* the correction scale kappa and the number of training steps taken */
 int sr_guess_kappa(const sr_guess *g, double *kappa, int *trained);
+/* Measurement: the average duration of one layer's kernel over `reps` launches on the handle's
+   current tensors (HIP events on the handle's stream; the work tensors it writes are overwritten).
+   which 0: the convolution of the forward pass; 1: its adjoint (the data gradient); 2: the weight
+   gradient with the bias gradient.  flops = 2 * 27 * cin * cout * voxels of the layer's grid, the
+   multiply-adds of one 3x3x3 convolution (the same count for the three). */
+int sr_guess_time_layer(sr_guess *g, int layer, int which, int reps, double *avg_ms, double *flops);
 
 #ifdef __cplusplus
 }

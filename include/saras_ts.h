@@ -121,6 +121,16 @@ int sr_ts_attach_guess(sr_ts *ts, sr_guess *g, int train);
  * sr_mg_history of the last solve or sr_mg_info.                                                */
 int sr_ts_pressure_solver(sr_ts *ts, sr_mg **mg);
 
+/* Measurement (no reference counterpart): the algorithmic HBM bytes of one launch of a step kernel
+   and its average duration over `reps` launches on the handle's fields (HIP events on the handle's
+   stream; the kernels overwrite what they write, so a handle timed this way is for measurement).
+   kernel 0: the three nseRHS kernels (40 B per face node: V_c, the two other components, P read,
+   H_c written); 1: one Jacobi sweep of Vx (24 B per padded entry); 2: its residual maximum (16 B per
+   bulk node); 3: divergence (32 B per node); 4: the three projection updates (24 B per face node);
+   5: the energy reduction (24 B per node). */
+int sr_ts_kernel_bytes(const sr_ts *ts, int kernel, double *bytes);
+int sr_ts_time_kernel(sr_ts *ts, int kernel, int reps, double *avg_ms);
+
 /* ---- scalar_d3 (src/solvers/scalar/scalar_d3.cc) ------------------------------------------- */
 /* the constructor's state (scalar_d3.cc:85-107: P = 1, T = 0, V = 0) with the temperature
  * perturbed on the core by amp (2u - 1), u uniform from SplitMix64(seed) in C order; then the

@@ -175,6 +175,7 @@ SIGNATURES = [
     ("sr_guess_kappa", C.c_int, [_vp, _dp, _ip]),
     ("sr_guess_extrapolate", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sr_guess_set_policy", C.c_int, [_vp, C.c_int]),
+    ("sr_guess_time_layer", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     ("sr_fgrid_create", C.c_int, [C.POINTER(SrFgridDesc), C.POINTER(C.c_void_p)]),
     ("sr_fgrid_destroy", C.c_int, [_vp]),
     ("sr_fgrid_dims", C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
@@ -192,6 +193,8 @@ SIGNATURES = [
     ("sr_jacobi_visc", C.c_int, [_vp, C.c_int, C.POINTER(SrJacobiDesc), C.POINTER(SrBc), _vp, _vp, _vp, _ip]),
     ("sr_probe_gather", C.c_int, [C.c_int, C.POINTER(C.c_void_p), _ip, C.c_int, _ip, _dp]),
     ("sr_ts_field_ptr", C.c_int, [_vp, C.c_int, C.POINTER(C.c_void_p)]),
+    ("sr_ts_kernel_bytes", C.c_int, [_vp, C.c_int, _dp]),
+    ("sr_ts_time_kernel", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
 ]
 
 _lib = None

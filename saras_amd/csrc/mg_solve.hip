@@ -27,6 +27,19 @@ using namespace sr;
         if (_r != SR_OK) return _r;                                                                   \
     } while (0)
 
+// the 32-byte read-back of the cycle's reduced quantities into the pinned host mirror; a failure
+// is kept in m->body_err (the call may be recorded into a graph, where it cannot return one)
+static void read_back(sr_mg *m) {
+    hipError_t e = hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+    if (e != hipSuccess && m->body_err == hipSuccess) m->body_err = e;
+}
+
+// before every cycle: the host mirror holds NaN until the cycle's read-back lands, so a read-back
+// that did not happen shows as a solve that never meets its stop rule, not as a stale norm
+static void poison_hres(sr_mg *m) {
+    for (int i = 0; i < 4; i++) m->hres[i] = std::nan("");
+}
+
 static bool is_singular(const sr_mg *m) {
     return !(m->d.testPoisson && !(m->d.xPer && m->d.yPer && m->d.zPer));
 }
@@ -84,7 +97,7 @@ static void cycle_fp64(sr_mg *m) {
     // a Gauss-Seidel cycle ends on a colour-1 half sweep of level 0, which leaves the colour-1
     // residual zero (up to the rounding of f - Lu): the norm is taken over colour 0
     norm_level0(m, m->d.gsSmooth && m->d.postSmooth > 0 ? 0 : -1);
-    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+    read_back(m);
 }
 
 static bool graph_ok(const sr_mg *m) {
@@ -99,9 +112,13 @@ static bool capture(sr_mg *m, void (*body)(sr_mg *), hipGraph_t &g, hipGraphExec
         g = nullptr;
         ex = nullptr;
         if (hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) break;
+        m->body_err = hipSuccess;
         body(m);
         hipError_t e1 = hipStreamEndCapture(m->stream, &g);
-        if (e1 == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) return true;
+        bool recorded = m->body_err == hipSuccess;
+        m->body_err = hipSuccess;
+        if (e1 == hipSuccess && recorded && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess)
+            return true;
         (void)hipGetLastError();
         if (g) (void)hipGraphDestroy(g);
         g = nullptr;
@@ -113,20 +130,24 @@ static bool capture(sr_mg *m, void (*body)(sr_mg *), hipGraph_t &g, hipGraphExec
     return false;
 }
 
+// a body run eagerly: its launches' and its read-back's status
+static int run_eager(sr_mg *m, void (*body)(sr_mg *)) {
+    m->body_err = hipSuccess;
+    body(m);
+    hipError_t e = m->body_err;
+    m->body_err = hipSuccess;
+    HIPCHK(e);
+    HIPCHK(hipGetLastError());
+    return SR_OK;
+}
+
 static int run_cycle(sr_mg *m, int kind, void (*body)(sr_mg *)) {
-    if (!graph_ok(m) || m->graphs_off) {
-        body(m);
-        HIPCHK(hipGetLastError());
-        return SR_OK;
-    }
+    poison_hres(m);
+    if (!graph_ok(m) || m->graphs_off) return run_eager(m, body);
     if (m->graph_kind != kind) {
         if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
         if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
-        if (!capture(m, body, m->graph, m->graph_exec)) {
-            body(m);
-            HIPCHK(hipGetLastError());
-            return SR_OK;
-        }
+        if (!capture(m, body, m->graph, m->graph_exec)) return run_eager(m, body);
         m->graph_kind = kind;
     }
     HIPCHK(hipGraphLaunch(m->graph_exec, m->stream));
@@ -175,7 +196,7 @@ static void mixed_body_ab(sr_mg *m) {
     vcycle_fp32(m);
     launch_outer_residual(V.L, V.u, V.u32, V.f, V.t, V.f32, V.c64[0], V.c64[1], V.c64[2], pressure_bc(m, 0, 0),
                           m->part2, m->fterms, m->dres, m->stream);
-    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+    read_back(m);
 }
 
 static void mixed_body_ba(sr_mg *m) {
@@ -183,18 +204,15 @@ static void mixed_body_ba(sr_mg *m) {
     vcycle_fp32(m);
     launch_outer_residual(V.L, V.t, V.u32, V.f, V.u, V.f32, V.c64[0], V.c64[1], V.c64[2], pressure_bc(m, 0, 0),
                           m->part2, m->fterms, m->dres, m->stream);
-    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, m->stream);
+    read_back(m);
 }
 
 static int run_graph2(sr_mg *m, int which) {
     void (*body)(sr_mg *) = which == 0 ? mixed_body_ab : mixed_body_ba;
     hipGraphExec_t &ex = m->graph_mixed[which];
+    poison_hres(m);
     if (!ex && !m->graphs_off) capture(m, body, m->graph_mixed_g[which], ex);
-    if (!ex) {
-        body(m);
-        HIPCHK(hipGetLastError());
-        return SR_OK;
-    }
+    if (!ex) return run_eager(m, body);
     HIPCHK(hipGraphLaunch(ex, m->stream));
     return SR_OK;
 }
@@ -266,7 +284,7 @@ static void gcr_body(sr_mg *m, int cur) {
     launch_gcr_step(V.L, x_in, V.u32, z, m->gcoef, x_out, z_next, pressure_bc(m, 0, 0), error_bc(m, 0, 0), s);
     launch_outer_residual(V.L, x_out, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
                           pressure_bc(m, 0, 0), m->part2, m->fterms, m->dres, s);
-    (void)hipMemcpyAsync(m->hres, m->dres, sizeof(double) * 4, hipMemcpyDeviceToHost, s);
+    read_back(m);
 }
 static void gcr_body_0(sr_mg *m) { gcr_body(m, 0); }
 static void gcr_body_1(sr_mg *m) { gcr_body(m, 1); }
@@ -274,12 +292,9 @@ static void gcr_body_1(sr_mg *m) { gcr_body(m, 1); }
 static int run_gcr_graph(sr_mg *m, int which) {
     void (*body)(sr_mg *) = which == 0 ? gcr_body_0 : gcr_body_1;
     hipGraphExec_t &ex = m->graph_gcr[which];
+    poison_hres(m);
     if (!ex && !m->graphs_off) capture(m, body, m->graph_gcr_g[which], ex);
-    if (!ex) {
-        body(m);
-        HIPCHK(hipGetLastError());
-        return SR_OK;
-    }
+    if (!ex) return run_eager(m, body);
     HIPCHK(hipGraphLaunch(ex, m->stream));
     return SR_OK;
 }

@@ -38,6 +38,7 @@ struct sr_mg {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t sw0 = nullptr, sw1 = nullptr;   // around the finest pre-smoothing sweeps of a cycle
     bool marks_off = false;
+    hipError_t body_err = hipSuccess;   // first failed call inside a cycle body (captured or eager)
     bool graphs_off = false;
     double *wall[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *wvec[3] = {nullptr, nullptr, nullptr};

@@ -719,6 +719,93 @@ extern "C" int sr_ts_attach_guess(sr_ts *t, sr_guess *g, int train) {
     return SR_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// measurement
+// ------------------------------------------------------------------------------------------------
+static Bulk bulk_of(const sr_ts *t, const FShape &f) {
+    int per[3] = {t->d.xPer, t->d.yPer, t->d.zPer};
+    Bulk B{};
+    for (int k = 0; k < 3; k++) {
+        B.lo[k] = 0;
+        B.hi[k] = f.n[k] - 1;
+        if (f.s[k] && !per[k]) {
+            B.lo[k] = 1;
+            B.hi[k] = f.n[k] - 2;
+        }
+    }
+    return B;
+}
+
+static long core_n(const FShape &f) { return (long)f.n[0] * f.n[1] * f.n[2]; }
+
+extern "C" int sr_ts_kernel_bytes(const sr_ts *t, int kernel, double *bytes) {
+    if (!t || !bytes) return set_err(SR_ERR_ARG, "null argument");
+    const double w = 8.0;
+    double b = 0.0;
+    switch (kernel) {
+        case 0:   // nseRHS of the three components: V_c, the two other components' slices, P read, H_c written
+            for (int c = 0; c < 3; c++) b += 5.0 * w * (double)core_n(t->fv[c]);
+            break;
+        case 1: b = 3.0 * w * (double)fsize(t->fv[0]); break;           // Jacobi sweep of Vx: V, R read, G written
+        case 2: {                                                       // its residual over the bulk: V, R read
+            Bulk B = bulk_of(t, t->fv[0]);
+            b = 2.0 * w * (double)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
+            break;
+        }
+        case 3: b = 4.0 * w * (double)core_n(t->fp); break;            // divergence: three faces read, div written
+        case 4:                                                         // projection: Pp read, V_c read and written
+            for (int c = 0; c < 3; c++) b += 3.0 * w * (double)core_n(t->fv[c]);
+            break;
+        case 5: b = 3.0 * w * (double)core_n(t->fp); break;            // energy: three faces read
+        default: return set_err(SR_ERR_ARG, "unknown kernel id");
+    }
+    *bytes = b;
+    return SR_OK;
+}
+
+extern "C" int sr_ts_time_kernel(sr_ts *t, int kernel, int reps, double *avg_ms) {
+    if (!t || !avg_ms || reps < 1 || kernel < 0 || kernel > 5) return set_err(SR_ERR_ARG, "bad arguments");
+    const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
+    const double visc = t->d.scalar ? t->nu : 1.0 / t->d.Re;
+    JacobiCoef J = jacobi_coef(t);
+    J.form = t->d.scalar ? 1 : 0;
+    J.vis = t->nu;
+    for (int k = 0; k < 3; k++) J.ms[k] = t->fv[0].s[k];
+    Bulk B = bulk_of(t, t->fv[0]);
+    auto launch = [&]() {
+        switch (kernel) {
+            case 0:
+                for (int c = 0; c < 3; c++)
+                    ts_build_rhs(c, t->fv, t->fp, t->M, Vc, t->P, t->H[c], t->h, visc, t->d.dt,
+                                 t->d.scalar ? t->T : nullptr, t->Fb, t->s);
+                break;
+            case 1: ts_jacobi(t->fv[0], t->M, B, t->V[0], t->H[0], t->G[0], J, t->s); break;
+            case 2: ts_jacobi_res(t->fv[0], t->M, B, t->V[0], t->H[0], J, t->part, t->dmax, t->s); break;
+            case 3: ts_divergence(t->fp, t->fv, t->M, Vc, t->divV, t->h, 1.0, 0, t->s); break;
+            case 4:
+                for (int c = 0; c < 3; c++) ts_project(c, t->fv[c], t->fp, t->M, t->Pp, t->V[c], t->h[c], t->d.dt, t->s);
+                break;
+            case 5: ts_energy(t->fp, t->fv, t->M, Vc, t->h, t->part, t->dred, t->s); break;
+        }
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch();
+    HIPCHK(hipEventRecord(e0, t->s));
+    for (int r = 0; r < reps; r++) launch();
+    HIPCHK(hipEventRecord(e1, t->s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipError_t e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(e);
+    HIPCHK(hipGetLastError());
+    *avg_ms = (double)ms / reps;
+    return SR_OK;
+}
+
 extern "C" int sr_ts_pressure_solver(sr_ts *t, sr_mg **mg) {
     if (!t || !mg) return set_err(SR_ERR_ARG, "null argument");
     *mg = t->mg;

@@ -324,6 +324,44 @@ int sr::guess_adam_device(sr_guess *g) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// measurement: one layer's kernels timed with HIP events on the handle's stream
+// ------------------------------------------------------------------------------------------------
+extern "C" int sr_guess_time_layer(sr_guess *g, int l, int which, int reps, double *avg_ms, double *flops) {
+    if (!g || l < 0 || l >= SR_GUESS_LAYERS || which < 0 || which > 2 || reps < 1 || !avg_ms)
+        return set_err(SR_ERR_ARG, "bad arguments");
+    // the input and output tensors of each layer in the forward pass (guess_forward_device)
+    static const int in_t[SR_GUESS_LAYERS] = {X0, A0, X1, A1, X2, A2, X3, A3, U2, U1, U0, G0};
+    static const int out_t[SR_GUESS_LAYERS] = {A0, S0, A1, S1, A2, S2, A3, BB, G2, G1, G0, -1};
+    const LayerSpec &L = kLayers[l];
+    auto launch = [&]() -> int {
+        if (which == 0) {
+            void *out = out_t[l] >= 0 ? (void *)g->T[out_t[l]] : (void *)g->y;
+            return conv(g, l, g->T[in_t[l]], out, l == SR_GUESS_LAYERS - 1 ? CONV_F32_CH0 : CONV_BF16_RELU);
+        }
+        if (which == 1) return conv_adj(g, l, g->tM, g->tF);
+        return wgrad(g, l, g->tM, g->T[in_t[l]]);
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int r = launch();
+    if (r == SR_OK) r = launch();
+    if (r == SR_OK && hipEventRecord(e0, g->s) != hipSuccess) r = set_err(SR_ERR_HIP, "event record");
+    for (int i = 0; i < reps && r == SR_OK; i++) r = launch();
+    float ms = 0.f;
+    if (r == SR_OK && (hipEventRecord(e1, g->s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        r = set_err(SR_ERR_HIP, "event timing");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r != SR_OK) return r;
+    HIPCHK(hipGetLastError());
+    *avg_ms = (double)ms / reps;
+    if (flops) *flops = 2.0 * 27.0 * L.cin * L.cout * (double)g->lv[L.level].nvox();
+    return SR_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
 // host-array entry points
 // ------------------------------------------------------------------------------------------------
 extern "C" int sr_guess_forward(sr_guess *g, const float *x0, float *y) {

@@ -121,6 +121,12 @@ class GuessNet:
         _L.check(self.L.sr_guess_extrapolate(self.h, C.c_void_p(p1_ptr), C.c_void_p(p2_ptr) if p2_ptr else None,
                                              C.c_void_p(out_ptr)))
 
+    def time_layer(self, layer, which, reps=5):
+        """(average ms, flops) of one layer's convolution (which 0), adjoint (1) or weight gradient (2)"""
+        ms, fl = C.c_double(), C.c_double()
+        _L.check(self.L.sr_guess_time_layer(self.h, layer, which, reps, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
     def set_policy(self, policy):
         """0: best guess by residual, network every time; 1: learned always; 2: adaptive network use;
         3: never the network (base guess alone)."""

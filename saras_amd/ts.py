@@ -67,6 +67,16 @@ class Hydro:
             pass
 
     # -- fields -----------------------------------------------------------------------------
+    def kernel_bytes(self, kernel):
+        b = C.c_double()
+        _L.check(self.L.sr_ts_kernel_bytes(self.h, kernel, C.byref(b)))
+        return b.value
+
+    def time_kernel(self, kernel, reps=10):
+        ms = C.c_double()
+        _L.check(self.L.sr_ts_time_kernel(self.h, kernel, reps, C.byref(ms)))
+        return ms.value
+
     def device_ptr(self, which):
         """Device address of a field's padded array (valid while the handle lives)."""
         p = C.c_void_p()

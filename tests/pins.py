@@ -237,3 +237,116 @@ def pad_mode1_weighted_transpose(library=None):
 
     want = (Pm.T * trap(nf)[None, :]) / trap(nc)[:, None] / 8.0
     assert np.max(np.abs(R - want)) < 1e-15, np.max(np.abs(R - want))
+
+
+# ------------------------------------------------------------------------------------------------
+# field operators on caller arrays (or_fd_*): closed forms on polynomials in the computational
+# coordinates.  Node coordinates are xi = i h, face coordinates xi = (i + 1/2) h (index -1..n);
+# central differences are exact on quadratics, two- and four-point face averages on linear data.
+# ------------------------------------------------------------------------------------------------
+_FD_GRID = dict(ind=(4, 3, 4), L=(1.0, 2.0, 1.0), domain="NPN", mesh="DUD", beta=(1.1, 1.0, 1.3))
+
+
+def _fd_oracle(library):
+    return po.OracleTS(po.ts_params(**_FD_GRID), library=library)
+
+
+def _fd_coords(o, loc):
+    """computational coordinates of every padded entry of location loc (0 node, 1..3 faces)"""
+    sh = o.loc_shape(loc)
+    h = [1.0 / (1 << n) for n in (o.params.xInd, o.params.yInd, o.params.zInd)]
+    ax = []
+    for d in range(3):
+        face = loc == d + 1
+        ax.append(h[d] * (np.arange(sh[d]) - 1 + (0.5 if face else 0.0)))
+    return np.meshgrid(*ax, indexing="ij")
+
+
+def _fd_metric(o, loc, d, q):
+    node = loc == 0 or loc - 1 != d
+    return o.metric(d, node, q)[:o.loc_shape(loc)[d]].reshape([-1 if a == d else 1 for a in range(3)])
+
+
+def fd_diff_quadratic(library=None):
+    """computeDiff of F = sum_d c_d xi_d^2 adds sum_d (xi_dd 2 c_d xi_d + 0.5 xi_d^2 2 c_d) to the
+    core of H and leaves H's pads as they were (a node field and the x faces)."""
+    o = _fd_oracle(library)
+    rng = np.random.default_rng(31)
+    for loc in (0, 1):
+        X = _fd_coords(o, loc)
+        c = (1.0, -2.0, 0.5)
+        F = sum(c[d] * X[d] ** 2 for d in range(3))
+        H0 = rng.standard_normal(F.shape)
+        H = H0.copy()
+        o.fd_diff(loc, F, H)
+        add = sum(_fd_metric(o, loc, d, 2) * 2 * c[d] * X[d] + 0.5 * _fd_metric(o, loc, d, 3) * 2 * c[d]
+                  for d in range(3)) * np.ones(F.shape)
+        exp = H0.copy()
+        exp[1:-1, 1:-1, 1:-1] += add[1:-1, 1:-1, 1:-1]
+        assert np.allclose(H, exp, rtol=1e-11, atol=1e-10 * np.max(np.abs(add))), loc
+    o.close()
+
+
+def fd_nlin_linear(library=None):
+    """computeNLin of F = sum_d a_d xi_d advected by V_d = U_d + b_d xi_d + g_d xi_e (e the next
+    direction) subtracts sum_d Vbar_d xi_d,x a_d, Vbar_d the same linear function at the location
+    of F: the interpolation slices are averages of face values around that location."""
+    o = _fd_oracle(library)
+    a, U, b, g = (0.7, -1.3, 2.1), (1.5, -0.25, 0.8), (0.3, -0.6, 0.45), (-0.2, 0.35, 0.15)
+    V = []
+    for d in range(3):
+        Xd = _fd_coords(o, d + 1)
+        V.append(U[d] + b[d] * Xd[d] + g[d] * Xd[(d + 1) % 3])
+    for loc in (0, 1, 2, 3):
+        X = _fd_coords(o, loc)
+        F = sum(a[d] * X[d] for d in range(3))
+        H = np.zeros(F.shape)
+        o.fd_nlin(loc, F, V, H)
+        sub = sum((U[d] + b[d] * X[d] + g[d] * X[(d + 1) % 3]) * _fd_metric(o, loc, d, 1) * a[d] for d in range(3))
+        exp = np.zeros(F.shape)
+        exp[1:-1, 1:-1, 1:-1] = -sub[1:-1, 1:-1, 1:-1]
+        assert np.allclose(H, exp, rtol=1e-11, atol=1e-12 * np.max(np.abs(sub))), loc
+    o.close()
+
+
+def fd_gradient_divergence_linear(library=None):
+    """gradient of F = sum_d a_d xi_d is xi_c(face) a_c on each face core and leaves the rest; the
+    divergence of V_c = b_c xi_c is sum_c xi_c(node) b_c on the node core and 0 on its pads."""
+    o = _fd_oracle(library)
+    X = _fd_coords(o, 0)
+    a = (0.3, -0.9, 1.7)
+    F = sum(a[d] * X[d] for d in range(3))
+    G = [np.full(o.loc_shape(c + 1), 7.0) for c in range(3)]
+    o.fd_gradient(F, G)
+    for c in range(3):
+        exp = np.full(G[c].shape, 7.0)
+        val = _fd_metric(o, c + 1, c, 1) * a[c] * np.ones(G[c].shape)
+        exp[1:-1, 1:-1, 1:-1] = val[1:-1, 1:-1, 1:-1]
+        assert np.allclose(G[c], exp, rtol=1e-12, atol=1e-13), c
+    b = (1.1, 0.4, -2.2)
+    V = [b[c] * _fd_coords(o, c + 1)[c] for c in range(3)]
+    div = o.fd_divergence(V)
+    val = sum(_fd_metric(o, 0, c, 1) * b[c] for c in range(3)) * np.ones(div.shape)
+    exp = np.zeros(div.shape)
+    exp[1:-1, 1:-1, 1:-1] = val[1:-1, 1:-1, 1:-1]
+    assert np.allclose(div, exp, rtol=1e-12, atol=1e-13)
+    o.close()
+
+
+def fd_bc_rules(library=None):
+    """Wall rules on Vx of the NPN grid: x walls (faces, FC) Dirichlet 0.5 at x- (mean of image and
+    first face), Neumann at x+ (image = last face); y periodic (syncData: ghost = the core layer one
+    in from the far side, the periodic node duplicated); z walls (nodes, CC) Dirichlet 1.5 at z- (the
+    end node set), Neumann at z+ (end node = its neighbour)."""
+    o = _fd_oracle(library)
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal(o.loc_shape(1))
+    o.fd_bc(1, [1, 2, 0, 0, 1, 2], [0.5, 0.0, 0.0, 0.0, 1.5, 0.0], F)
+    ny = F.shape[1] - 2
+    assert np.allclose((F[0, 2:-2, 2:-2] + F[1, 2:-2, 2:-2]) / 2, 0.5, atol=1e-14)
+    assert np.array_equal(F[-1, 2:-2, 2:-2], F[-2, 2:-2, 2:-2])
+    assert np.array_equal(F[2:-2, 0, 2:-2], F[2:-2, ny - 1, 2:-2])      # F(-1) = F(M - 1), M = ny - 1
+    assert np.array_equal(F[2:-2, -1, 2:-2], F[2:-2, 2, 2:-2])          # F(M + 1) = F(1)
+    assert np.all(F[:, :, 1] == 1.5)
+    assert np.array_equal(F[:, :, -2], F[:, :, -3])
+    o.close()

@@ -36,6 +36,20 @@ MUTATIONS = [
      pins.pad_mode1_cosine_modes),
     ("mg_oracle.c", "static void consistent_pads", "a[IX(L, i, My + 1, k)] = a[IX(L, i, My - 1, k)];",
      "a[IX(L, i, My + 1, k)] = a[IX(L, i, My, k)];", pins.pad_mode1_cosine_modes),
+    ("field_oracle.c", "void or_fd_diff", "H[FI(f, i, j, k)] += t->deriv[FI(f, i, j, k)];",
+     "H[FI(f, i, j, k)] = t->deriv[FI(f, i, j, k)];", pins.fd_diff_quadratic),
+    ("field_oracle.c", "static void d2(", "out[c] = xx * tmp + 0.5 * x2 * sec;", "out[c] = xx * tmp + x2 * sec;",
+     pins.fd_diff_quadratic),
+    ("field_oracle.c", "void or_fd_nlin", "off[1][d] = -1;", "off[1][d] = 1;", pins.fd_nlin_linear),
+    ("field_oracle.c", "static int slices", "off[2][c] = 1;", "off[2][c] = -1;", pins.fd_nlin_linear),
+    ("field_oracle.c", "static void d1(", "out[c] = met(t, d, f->s[d], 1, r) * v;", "out[c] = met(t, d, 1, 1, r) * v;",
+     pins.fd_nlin_linear),
+    ("field_oracle.c", "static void gradient(", "met(t, c, 0, 1, r)", "met(t, c, 1, 1, r)",
+     pins.fd_gradient_divergence_linear),
+    ("field_oracle.c", "void or_fd_divergence", "V[1][FI(&t->sV[1], i, j - 1, k)]", "V[1][FI(&t->sV[1], i, j + 1, k)]",
+     pins.fd_gradient_divergence_linear),
+    ("field_oracle.c", "void or_fd_bc", "int kind = kinds[w] == 1 ? (node ? 2 : 3) : (node ? 4 : 5);",
+     "int kind = kinds[w] == 1 ? (node ? 3 : 2) : (node ? 4 : 5);", pins.fd_bc_rules),
 ]
 
 
