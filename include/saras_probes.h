@@ -27,6 +27,16 @@ extern "C" {
 int sr_probe_gather(int nfields, const double *const *fields, const int *dims, int nprobes, const int *idx,
                     double *out);
 
+/* No counterpart in the reference.  sr_probe_gather for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+   gather and results, with the probe list and the values staged in a static device buffer of
+   SR_PROBE_NOALLOC_MAX probes (368 KiB, held by every GPU the library runs on until the process
+   ends) instead of buffers allocated per call.  More probes than that is SR_ERR_ARG.  Every call
+   on a GPU shares that GPU's buffer: calls that overlap on one GPU (several host threads) race. */
+#define SR_PROBE_NOALLOC_MAX 4096
+int sr_probe_gather_noalloc(int nfields, const double *const *fields, const int *dims, int nprobes, const int *idx,
+                            double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -192,6 +192,7 @@ SIGNATURES = [
     ("sr_bc_fill", C.c_int, [_vp, C.c_int, C.POINTER(SrBc), _vp]),
     ("sr_jacobi_visc", C.c_int, [_vp, C.c_int, C.POINTER(SrJacobiDesc), C.POINTER(SrBc), _vp, _vp, _vp, _ip]),
     ("sr_probe_gather", C.c_int, [C.c_int, C.POINTER(C.c_void_p), _ip, C.c_int, _ip, _dp]),
+    ("sr_probe_gather_noalloc", C.c_int, [C.c_int, C.POINTER(C.c_void_p), _ip, C.c_int, _ip, _dp]),
     ("sr_ts_field_ptr", C.c_int, [_vp, C.c_int, C.POINTER(C.c_void_p)]),
     ("sr_ts_kernel_bytes", C.c_int, [_vp, C.c_int, _dp]),
     ("sr_ts_time_kernel", C.c_int, [_vp, C.c_int, C.c_int, _dp]),

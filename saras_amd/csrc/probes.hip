@@ -2,6 +2,7 @@
 // as one gather launch over (probe, field) pairs and one transfer of the values to the host.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/saras_mg.h"
@@ -59,6 +60,57 @@ int check_probe_args(int nfields, const double *const *fields, const int *dims, 
 }
 
 }  // namespace
+
+// Staging of sr_probe_gather_noalloc: the probe list (3 ints per probe) followed, at a 256-byte
+// boundary, by the gathered values (SR_PROBE_MAX_FIELDS doubles per probe).  Every GPU holds its
+// own copy for the life of the process; every call on a GPU shares it.
+static constexpr size_t kScratchIdxBytes = ((size_t)SR_PROBE_NOALLOC_MAX * 3 * sizeof(int) + 255) / 256 * 256;
+static constexpr size_t kScratchBytes = kScratchIdxBytes + (size_t)SR_PROBE_NOALLOC_MAX * SR_PROBE_MAX_FIELDS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// device address of `scratch` per device, taken once per device with that device current (the
+// array's name is a host-side stand-in and is never used as an address on the host)
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_ptr(unsigned char **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_err(SR_ERR_STATE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(SR_ERR_HIP, "no device address for the probe scratch");
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SR_OK;
+}
+
+extern "C" int sr_probe_gather_noalloc(int nfields, const double *const *fields, const int *dims, int nprobes,
+                                       const int *idx, double *out) {
+    ProbeFields P{};
+    int r = check_probe_args(nfields, fields, dims, nprobes, idx, out, P);
+    if (r != SR_OK) return r;
+    if (nprobes > SR_PROBE_NOALLOC_MAX)
+        return set_err(SR_ERR_ARG, "more than SR_PROBE_NOALLOC_MAX probes for the allocation-free gather");
+    if (nprobes == 0) return SR_OK;
+    unsigned char *base = nullptr;
+    r = scratch_ptr(&base);
+    if (r != SR_OK) return r;
+    int *didx = (int *)base;
+    double *dout = (double *)(base + kScratchIdxBytes);
+    const int total = nprobes * nfields;
+    // every entry the kernel reads is written first (the probe list) and every value read back is
+    // written by the kernel, so nothing has to start at zero
+    HIPCHK(hipMemcpy(didx, idx, sizeof(int) * 3 * (size_t)nprobes, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe_gather, dim3((total + 255) / 256), dim3(256), 0, 0, P, nfields, nprobes, didx, dout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dout, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost));
+    return SR_OK;
+}
 
 extern "C" int sr_probe_gather(int nfields, const double *const *fields, const int *dims, int nprobes, const int *idx,
                                double *out) {
