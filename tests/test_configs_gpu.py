@@ -11,7 +11,8 @@
 Bounds: the FP64 solve matches the oracle's to 1e-12 relative (same cycle order); refinement follows
 the FP64 iterates (L2 difference <= 1e-10 relative); GCR stops at another point inside the same
 1e-10 residual ball (L2 difference <= 1e-8 relative, bounded by the residual times the operator's
-conditioning); the time steps match at 1e-12 like the 33^3 cases."""
+conditioning); the time-step fields match at 1e-12 like the 33^3 cases, the energy to the rounding
+bound of its sequential sum."""
 import numpy as np
 import pytest
 
@@ -110,8 +111,11 @@ def test_c3_taylor_green_129_cubed_steps():
         for c in range(3):
             ok, err, s = rel_close(g.download(c), o.array(c), 1e-12, scale=vs)
             assert ok, (step, c, err)
+        # the energy is a sum over 129^3 = 2.1e6 nodes: the reference (and the oracle) add them one
+        # after the other (tseries.cc:158-169), whose rounding is bounded by n eps |E| = 2.4e-10 |E|;
+        # the device adds them as a tree.  The bar is that bound (measured 4e-11 at the first step).
         e_o, e_g = o.energy(), g.energy()
-        assert abs(e_o - e_g) <= 1e-12 * abs(e_o), (step, e_o, e_g)
+        assert abs(e_o - e_g) <= 129 ** 3 * 2.2e-16 * abs(e_o), (step, e_o, e_g)
     # the energy decays at the viscous rate of the vortex, E ~ E0 exp(-6 t / Re) at early times
     assert abs(e_g / e0 - np.exp(-6 * 3e-3 / 1600.0)) < 1e-4
     g.close()

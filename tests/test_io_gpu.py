@@ -56,7 +56,8 @@ def test_run_writes_probes_and_restart_and_continues_from_it(tmp_path):
     assert len(plist) == 10 and len(lines) == 1 + 4 * 10            # t = 0, 0.002, 0.004, 0.006
     assert [l.split("\t")[0] for l in lines[1::10]] == ["0.0000", "0.0020", "0.0040", "0.0060"]
     # the last probe lines against the restart file written at the same time
-    fields = {n: np.zeros(s) for n, s in (("Vx", (17, 19, 19)), ("Vy", (19, 17, 19)), ("Vz", (19, 19, 17)),
+    # 17 nodes per direction: 16 faces; padded shapes
+    fields = {n: np.zeros(s) for n, s in (("Vx", (18, 19, 19)), ("Vy", (19, 18, 19)), ("Vz", (19, 19, 18)),
                                           ("P", (19, 19, 19)))}
     t_rs = restart.read(str(a / "output" / "restartFile.h5"), fields)
     assert abs(t_rs - 0.006) < 1e-12
