@@ -241,18 +241,33 @@ __global__ void __launch_bounds__(256) k_wgrad(NetGrid g, const bf16_t *__restri
     }
 }
 
-// dW[tap][co][ci] = sum over slots of the partials, in slot order
-__global__ void k_wgrad_final(const float *__restrict__ part, int cin, int cout, int nslots, float *__restrict__ dW) {
-    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = 27L * cout * cin;
-    if (e >= total) return;
-    int ci = (int)(e % cin), co = (int)((e / cin) % cout), tap = (int)(e / ((long)cin * cout));
-    int nib = (cin + 15) / 16, npair = (cout / 16) * nib;
-    int pair = (co / 16) * nib + ci / 16;
-    int idx = (co % 16) * 16 + (ci % 16);
+// dW[tap][co][ci] = sum over slots of the partials.  A block reduces 16 consecutive outputs (one
+// channel pair and tap, consecutive ci): 16 threads per output each add the slots j, j + 16, ... in
+// order, then one thread per output adds the 16 partial sums in order -- a fixed order, so the
+// result is the same on every run.
+__global__ void __launch_bounds__(256) k_wgrad_final(const float *__restrict__ part, int cin, int cout, int nslots,
+                                                     float *__restrict__ dW) {
+    const long total = 27L * cout * cin;
+    const int o = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const long e = (long)blockIdx.x * 16 + o;
     float s = 0.f;
-    for (int sl = 0; sl < nslots; sl++) s += part[(((long)sl * npair + pair) * 27 + tap) * 256 + idx];
-    dW[e] = s;
+    if (e < total) {
+        int ci = (int)(e % cin), co = (int)((e / cin) % cout), tap = (int)(e / ((long)cin * cout));
+        int nib = (cin + 15) / 16, npair = (cout / 16) * nib;
+        int pair = (co / 16) * nib + ci / 16;
+        int idx = (co % 16) * 16 + (ci % 16);
+        const float *p = part + ((long)pair * 27 + tap) * 256 + idx;
+        const long stride = (long)npair * 27 * 256;
+        for (int sl = j; sl < nslots; sl += 16) s += p[(long)sl * stride];
+    }
+    __shared__ float red[16][17];
+    red[j][o] = s;
+    __syncthreads();
+    if (threadIdx.x < 16 && e < total) {
+        float t = 0.f;
+        for (int q = 0; q < 16; q++) t += red[q][threadIdx.x];
+        dW[e] = t;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_bias_part(NetGrid g, int cout, const bf16_t *__restrict__ dy, int nslots,
@@ -274,12 +289,24 @@ __global__ void __launch_bounds__(256) k_bias_part(NetGrid g, int cout, const bf
     }
 }
 
-__global__ void k_bias_final(const float *__restrict__ part, int cout, int nslots, float *__restrict__ db) {
-    int ch = threadIdx.x;
-    if (ch >= cout) return;
+// db[ch] = sum over slots of the partials: 256 / cout threads per channel (cout divides 256) each
+// add the slots lane, lane + 256 / cout, ... in order, then one thread per channel adds those sums in
+// order (a fixed order, the same result on every run)
+__global__ void __launch_bounds__(256) k_bias_final(const float *__restrict__ part, int cout, int nslots,
+                                                    float *__restrict__ db) {
+    const int per = 256 / cout;
+    const int ch = threadIdx.x % cout, lane = threadIdx.x / cout;
     float s = 0.f;
-    for (int sl = 0; sl < nslots; sl++) s += part[(long)sl * cout + ch];
-    db[ch] = s;
+    if (lane < per)
+        for (int sl = lane; sl < nslots; sl += per) s += part[(long)sl * cout + ch];
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.f;
+        for (int q = 0; q < per; q++) t += red[q * cout + threadIdx.x];
+        db[threadIdx.x] = t;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -448,7 +475,8 @@ long wgrad_part_elems(int cin, int cout, const NetGrid &g) {
 #define WGRAD_CASE(CI, CO)                                                                                 \
     if (cin == CI && cout == CO) {                                                                         \
         hipLaunchKernelGGL((k_wgrad<CI, CO>), dim3(nslots, npair), dim3(256), 0, s, g, dy, x, part, nslots); \
-        hipLaunchKernelGGL(k_wgrad_final, dim3(nblk(27L * cin * cout)), dim3(256), 0, s, part, cin, cout, nslots, dW); \
+        hipLaunchKernelGGL(k_wgrad_final, dim3((unsigned)((27L * cin * cout + 15) / 16)), dim3(256), 0, s, part, cin, \
+                           cout, nslots, dW);                                                              \
         return 0;                                                                                          \
     }
 
@@ -475,7 +503,7 @@ int bias_slots(const NetGrid &g) {
 void launch_bias_grad(int cout, const NetGrid &g, const bf16_t *dy, float *part, float *db, hipStream_t s) {
     int nslots = bias_slots(g);
     hipLaunchKernelGGL(k_bias_part, dim3(nslots), dim3(256), 0, s, g, cout, dy, nslots, part);
-    hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(64), 0, s, part, cout, nslots, db);
+    hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(256), 0, s, part, cout, nslots, db);
 }
 
 void launch_restrict_bf16(const NetGrid &fine, const NetGrid &coarse, int C, const bf16_t *in, bf16_t *out,
