@@ -317,7 +317,10 @@ def rbc_bench(args):
                "rel_guess_median": float(np.median([r.rel_guess for r in rec_l])),
                "rel_guess_per_step": [float(f"{r.rel_guess:.3e}") for r in rec_l],
                "step_ms_mean": float(np.mean([r.step_ms for r in rec_l])),
-               "sweep0_ms_mean": float(np.mean([r.sweep0_ms for r in rec_l]))}
+               "sweep0_ms_mean": float(np.mean([r.sweep0_ms for r in rec_l])),
+               # Jacobi sweeps of solveVx/Vy/Vz/solveT per step: a change in a count shifts the
+               # pressure right-hand side by the Jacobi tolerance, off the history's span
+               "jacobi_iters_per_step": [list(r.jacobi_iters) for r in rec_l]}
     nu_end = run.nusselt()
     # one real solve of this workload for the CPU baseline: the last step's right-hand side from
     # the solution before it (the reference's warm start)

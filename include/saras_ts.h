@@ -75,6 +75,10 @@ typedef struct sr_ts_stats {
                                     /* cycles without meeting it and the step went on with the last */
                                     /* iterate, as mgSolve does (poisson.cc:139-149)               */
     double mg_residual;             /* sr_mg_stats.residual of the pressure solve                  */
+    double guess_rel_linear;        /* sr_guess_stats.rel_linear: the linear predictor's relative  */
+                                    /* residual (exact when the network ran, else its fit's estimate), */
+                                    /* -1 without one                                              */
+    double guess_rel_learned;       /* sr_guess_stats.rel_learned, -1 when the network did not run */
 } sr_ts_stats;
 
 /* hydro_d3::hydro_d3 (src/solvers/hydro/hydro_d3.cc:63-119) without the initial condition */
