@@ -204,7 +204,8 @@ def tgv_bench(args):
     # bytes; timed after the stepped state was read, since they overwrite what they write
     kernels = {}
     for kid, name in ((0, "k_build_rhs x3"), (1, "k_jacobi_v (Vx)"), (2, "k_jacobi_res (Vx)"),
-                      (3, "k_divergence"), (4, "k_project x3"), (5, "k_energy")):
+                      (6, "k_jacobi_vres (Vx: sweep + residual of its input)"), (3, "k_divergence"),
+                      (4, "k_project x3"), (5, "k_energy")):
         kms = h.time_kernel(kid, reps=20)
         kb = h.kernel_bytes(kid)
         gbs = kb / (kms * 1e-3) / 1e9
@@ -591,7 +592,7 @@ def main():
     ap.add_argument("--pad-mode", type=int, default=1, choices=[0, 1],
                     help="convection workloads: residual pads of the mixed solves' restriction (0 the reference's "
                          "zero pads, 1 mirror at the walls); the plain FP64 baseline keeps the reference's")
-    ap.add_argument("--history", type=int, default=6,
+    ap.add_argument("--history", type=int, default=8,
                     help="converged pairs kept by the learned guess's least-squares predictor (0..8)")
     args = ap.parse_args()
     if args.gpus != 1:

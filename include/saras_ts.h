@@ -131,7 +131,8 @@ int sr_ts_pressure_solver(sr_ts *ts, sr_mg **mg);
    kernel 0: the three nseRHS kernels (40 B per face node: V_c, the two other components, P read,
    H_c written); 1: one Jacobi sweep of Vx (24 B per padded entry); 2: its residual maximum (16 B per
    bulk node); 3: divergence (32 B per node); 4: the three projection updates (24 B per face node);
-   5: the energy reduction (24 B per node). */
+   5: the energy reduction (24 B per node); 6: one Jacobi sweep of Vx together with the residual
+   maximum of its input, the form the step runs (24 B per padded entry). */
 int sr_ts_kernel_bytes(const sr_ts *ts, int kernel, double *bytes);
 int sr_ts_time_kernel(sr_ts *ts, int kernel, int reps, double *avg_ms);
 

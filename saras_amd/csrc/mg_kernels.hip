@@ -17,19 +17,11 @@
 // reference quotient of poisson_d3.cc:140-147 after dividing numerator and denominator by 2 hx^2hy^2hz^2.
 #include "mg_kernels.h"
 #include "mg_state.h"
+#include "xcd.h"
 
 #include <hip/hip_runtime.h>
 
 namespace sr {
-
-// Workgroups are dealt round-robin to the 8 XCDs; remap so that consecutive logical blocks (which
-// touch neighbouring columns) run on the same XCD and share its L2.
-__device__ inline unsigned xcd_block() {
-    unsigned nb = gridDim.x, b = blockIdx.x;
-    unsigned per = nb >> 3;
-    if (b < per * 8u) return (b & 7u) * per + (b >> 3);
-    return b;
-}
 
 // Launch geometry over whole planes: logical block b covers slots [256 (b % bpp), +256) of plane
 // ip0 + b / bpp.  The plane index is wave-uniform (scalar division); the split of the in-plane

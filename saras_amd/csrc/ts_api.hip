@@ -46,6 +46,7 @@ struct sr_ts {
     double *scratch = nullptr;                    // derivative output of the parity entry point
     double *part = nullptr, *dred = nullptr, *hred = nullptr;
     unsigned long long *dmax = nullptr, *hmax = nullptr;
+    hipEvent_t jev = nullptr;                    // marks a Jacobi residual's read-back
     std::vector<GhostWave> vbc[3];               // imposeU/V/WBCs as ordered waves
     GhostWave sync_v[3]{}, sync_p{};             // syncData of the velocity components and of P
     sr_mg *mg = nullptr;
@@ -252,8 +253,11 @@ static int create_impl(const sr_ts_desc *desc, sr_ts *t) {
         ts_fill(t->Tg, np, 0.0, t->s);
         t->tbc = tbc_waves(d, t->fp);
     }
-    HIPCHK(hipMalloc((void **)&t->part,
-                     sizeof(double) * std::max<size_t>(2 * (size_t)ts_energy_parts(t->fp), ts_jacobi_res_parts())));
+    size_t nparts = std::max<size_t>(2 * (size_t)ts_energy_parts(t->fp), ts_jacobi_res_parts());
+    nparts = std::max<size_t>(nparts, (size_t)ts_jacobi_vres_parts(t->fp));
+    for (int c = 0; c < 3; c++) nparts = std::max<size_t>(nparts, (size_t)ts_jacobi_vres_parts(t->fv[c]));
+    HIPCHK(hipMalloc((void **)&t->part, sizeof(double) * nparts));
+    HIPCHK(hipEventCreateWithFlags(&t->jev, hipEventDisableTiming));
     HIPCHK(hipMalloc((void **)&t->dred, sizeof(double) * 4));
     HIPCHK(hipHostMalloc((void **)&t->hred, sizeof(double) * 4, hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&t->dmax, sizeof(unsigned long long)));
@@ -281,6 +285,7 @@ extern "C" int sr_ts_destroy(sr_ts *t) {
         if (t->G[c]) (void)hipFree(t->G[c]);
     }
     if (t->dmax) (void)hipFree(t->dmax);
+    if (t->jev) (void)hipEventDestroy(t->jev);
     if (t->hred) (void)hipHostFree(t->hred);
     if (t->hmax) (void)hipHostFree(t->hmax);
     if (t->mg) sr_mg_destroy(t->mg);
@@ -438,7 +443,10 @@ static JacobiCoef jacobi_coef(const sr_ts *t) {
 // Jacobi sweeps of (1 - dt vis lap / 2) F = R until the maximum residual over the bulk is below
 // cnTolerance; the reference aborts after (Nx-1)(Ny-1)(Nz-1) sweeps without convergence.  F and G
 // swap after every sweep (G is written everywhere: 0 outside the bulk, as the reference's guess
-// field), then the field's BCs are imposed.
+// field), then the field's BCs are imposed.  After the first sweep, one launch per iteration makes
+// the next iterate in G and the residual of the current one in F (the same loads serve both); the
+// next iterate gets its BCs while the host reads the residual, and is dropped once F has converged,
+// so F, the residuals and the sweep count are those of the sweep-then-check loop.
 static int jacobi(sr_ts *t, const FShape &f, double *&F, double *&G, const double *R, const JacobiCoef &J,
                   const std::vector<GhostWave> &bc, const char *what, int *iters) {
     int per[3] = {t->d.xPer, t->d.yPer, t->d.zPer};
@@ -454,14 +462,16 @@ static int jacobi(sr_ts *t, const FShape &f, double *&F, double *&G, const doubl
     long maxIterations = (long)(t->N[0] - 1) * (t->N[1] - 1) * (t->N[2] - 1);
     long iterCount = 0;
     int sweeps = 0;
+    ts_jacobi(f, t->M, B, F, R, G, J, t->s);
+    std::swap(F, G);
+    for (const GhostWave &w : bc) ts_ghost_wave(f, F, w, t->s);
     while (true) {
-        ts_jacobi(f, t->M, B, F, R, G, J, t->s);
-        std::swap(F, G);
-        for (const GhostWave &w : bc) ts_ghost_wave(f, F, w, t->s);
-        ts_jacobi_res(f, t->M, B, F, R, J, t->part, t->dmax, t->s);
+        ts_jacobi_vres(f, t->M, B, F, R, G, J, t->part, t->dmax, t->s);
         HIPCHK(hipMemcpyAsync(t->hmax, t->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, t->s));
-        HIPCHK(hipStreamSynchronize(t->s));
+        HIPCHK(hipEventRecord(t->jev, t->s));
+        for (const GhostWave &w : bc) ts_ghost_wave(f, G, w, t->s);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(t->jev));
         double maxError;
         unsigned long long bits = *t->hmax;
         std::memcpy(&maxError, &bits, sizeof(double));
@@ -471,6 +481,7 @@ static int jacobi(sr_ts *t, const FShape &f, double *&F, double *&G, const doubl
         if (iterCount > maxIterations)
             return set_err(SR_ERR_NOT_CONVERGED, std::string("Jacobi iterations for ") + what +
                                                      " did not converge within the iteration limit");
+        std::swap(F, G);
     }
     if (iters) *iters = sweeps;
     return SR_OK;
@@ -761,6 +772,8 @@ extern "C" int sr_ts_kernel_bytes(const sr_ts *t, int kernel, double *bytes) {
             for (int c = 0; c < 3; c++) b += 3.0 * w * (double)core_n(t->fv[c]);
             break;
         case 5: b = 3.0 * w * (double)core_n(t->fp); break;            // energy: three faces read
+        case 6: b = 3.0 * w * (double)fsize(t->fv[0]); break;           // Jacobi sweep of Vx with the residual of
+                                                                        // its input: V, R read, G written
         default: return set_err(SR_ERR_ARG, "unknown kernel id");
     }
     *bytes = b;
@@ -768,7 +781,7 @@ extern "C" int sr_ts_kernel_bytes(const sr_ts *t, int kernel, double *bytes) {
 }
 
 extern "C" int sr_ts_time_kernel(sr_ts *t, int kernel, int reps, double *avg_ms) {
-    if (!t || !avg_ms || reps < 1 || kernel < 0 || kernel > 5) return set_err(SR_ERR_ARG, "bad arguments");
+    if (!t || !avg_ms || reps < 1 || kernel < 0 || kernel > 6) return set_err(SR_ERR_ARG, "bad arguments");
     const double *Vc[3] = {t->V[0], t->V[1], t->V[2]};
     const double visc = t->d.scalar ? t->nu : 1.0 / t->d.Re;
     JacobiCoef J = jacobi_coef(t);
@@ -790,6 +803,7 @@ extern "C" int sr_ts_time_kernel(sr_ts *t, int kernel, int reps, double *avg_ms)
                 for (int c = 0; c < 3; c++) ts_project(c, t->fv[c], t->fp, t->M, t->Pp, t->V[c], t->h[c], t->d.dt, t->s);
                 break;
             case 5: ts_energy(t->fp, t->fv, t->M, Vc, t->h, t->part, t->dred, t->s); break;
+            case 6: ts_jacobi_vres(t->fv[0], t->M, B, t->V[0], t->H[0], t->G[0], J, t->part, t->dmax, t->s); break;
         }
     };
     hipEvent_t e0, e1;

@@ -31,6 +31,10 @@ struct JacobiCoef {
     double vis;
     int form;
     int ms[3];
+    // set by the launchers from the fields above: the grid-uniform quotients of the sweep and of the
+    // residual, so that a node divides once (the sweep's per-node denominator) instead of six times
+    double rA, rD;            // sweep: G = (a rA + R) / (1 + d rD)
+    double ih2[3], lapS;      // residual: V - lapS sum_d m_d (V(+1) - 2 V + V(-1)) ih2_d
 };
 
 // one ghost / wall assignment F(dst) = a F(src) + b
@@ -71,6 +75,11 @@ void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V
 int ts_jacobi_res_parts();
 void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R,
                    const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s);
+// G = one Jacobi sweep of V (as ts_jacobi) and the max |residual| of V itself over the bulk (as
+// ts_jacobi_res) in one pass; part holds ts_jacobi_vres_parts(f) doubles
+long ts_jacobi_vres_parts(const FShape &f);
+void ts_jacobi_vres(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
+                    const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s);
 void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s);
 void ts_divergence(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, double *out,
                    const double *h, double scale, int scaled, hipStream_t s);

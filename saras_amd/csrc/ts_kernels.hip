@@ -5,6 +5,7 @@
 //                  (vfield.cc:104-253), - grad P (sfield.cc:155-171), * dt, + V
 //   k_jacobi_v     one Jacobi sweep of solveVx/Vy/Vz (hydro_d3.cc:375-391) into the second buffer
 //   k_jacobi_res   the convergence residual of the same routines (hydro_d3.cc:396-410), max-reduced
+//   k_jacobi_vres  the next sweep and the residual of the current iterate from the same loads
 //   k_ghost_ops    a wave of ghost / wall operations (mpidata::syncData, lib/boundary/*.cc)
 //   k_divergence   vfield::divergence (vfield.cc:299-320), optionally scaled (mgRHS = div / dt)
 //   k_project      P += Pp and V -= dt grad Pp (hydro_d3.cc:354-362)
@@ -16,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ts_kernels.h"
+#include "xcd.h"
 
 namespace sr {
 
@@ -71,7 +73,8 @@ __global__ void __launch_bounds__(256) k_build_rhs(int c, FShape fc, FShape fx, 
     // one thread per padded entry: core entries get the right-hand side, pads a copy of V (the
     // reference forms nseRHS over the whole padded array, with zero terms outside the core)
     long total = (long)(fc.n[0] + 2) * (fc.n[1] + 2) * (fc.n[2] + 2);
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    long t = (long)lb * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int k = (int)(t % (fc.n[2] + 2)) - 1;
     long r = t / (fc.n[2] + 2);
@@ -157,7 +160,8 @@ __global__ void __launch_bounds__(256) k_jacobi_v(FShape f, Metrics M, Bulk B, c
     // reference's guess field is written on the bulk only and copied whole into V)
     unsigned n1 = (unsigned)(f.n[1] + 2), n2 = (unsigned)(f.n[2] + 2);
     unsigned total = (unsigned)(f.n[0] + 2) * n1 * n2;
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    unsigned t = lb * blockDim.x + threadIdx.x;
     if (t >= total) return;
     unsigned r = t / n2;
     int k = (int)(t - r * n2) - 1;
@@ -174,22 +178,26 @@ __global__ void __launch_bounds__(256) k_jacobi_v(FShape f, Metrics M, Bulk B, c
     double a = J.hy2hz2 * mx * (V[q + st[0]] + V[q - st[0]]) + J.hz2hx2 * my * (V[q + st[1]] + V[q - st[1]]) +
                J.hx2hy2 * mz * (V[q + 1] + V[q - 1]);
     double d = J.hy2hz2 * mx + J.hz2hx2 * my + J.hx2hy2 * mz;
-    if (J.form == 0)
-        G[q] = (a * J.dt / (J.hx2hy2hz2 * 2.0 * J.Re) + R[q]) / (1.0 + J.dt * (d) / (J.Re * J.hx2hy2hz2));
-    else
-        G[q] = (a * J.dt * J.vis / (J.hx2hy2hz2 * 2.0) + R[q]) / (1.0 + J.dt * J.vis * (d) / J.hx2hy2hz2);
+    G[q] = (a * J.rA + R[q]) / (1.0 + d * J.rD);
 }
 
-// max over the bulk of |V - 0.5 dt/Re lap V - R|: grid-stride over the bulk (32-bit index
-// arithmetic), one maximum per block into part; k_jacobi_res_final reduces them
+// max over the bulk of |V - 0.5 dt/Re lap V - R|: each logical block takes one contiguous range of
+// the bulk (32-bit index arithmetic), so that an XCD's blocks cover neighbouring planes; one maximum
+// per block into part; k_jacobi_res_final reduces them (a maximum: the same whatever the order)
 __global__ void __launch_bounds__(256) k_jacobi_res(FShape f, Metrics M, Bulk B, const double *__restrict__ V,
                                                     const double *__restrict__ R, JacobiCoef J,
                                                     double *__restrict__ part) {
     unsigned sz1 = (unsigned)(B.hi[1] - B.lo[1] + 1), sz2 = (unsigned)(B.hi[2] - B.lo[2] + 1);
     unsigned total = (unsigned)(B.hi[0] - B.lo[0] + 1) * sz1 * sz2;
     long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
+    const unsigned lb = xcd_block();
+    // range of a block: a whole number of 256-entry strips
+    const unsigned strips = (total + 255u) / 256u;
+    const unsigned per = (strips + gridDim.x - 1u) / gridDim.x;
+    const unsigned t0 = lb * per * 256u;
+    const unsigned t1 = min(total, t0 + per * 256u);
     double e = 0.0;
-    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    for (unsigned t = t0 + threadIdx.x; t < t1; t += 256u) {
         unsigned r = t / sz2;
         int k = (int)(t - r * sz2) + B.lo[2];
         unsigned i0 = r / sz1;
@@ -197,10 +205,9 @@ __global__ void __launch_bounds__(256) k_jacobi_res(FShape f, Metrics M, Bulk B,
         int i = (int)i0 + B.lo[0];
         long q = fidx(f, i, j, k);
         double mx = metv(M, 0, J.ms[0], 3, i), my = metv(M, 1, J.ms[1], 3, j), mz = metv(M, 2, J.ms[2], 3, k);
-        double lap = V[q] - (mx * (V[q + st[0]] - 2.0 * V[q] + V[q - st[0]]) / (J.hx * J.hx) +
-                             my * (V[q + st[1]] - 2.0 * V[q] + V[q - st[1]]) / (J.hy * J.hy) +
-                             mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) / (J.hz * J.hz)) * 0.5 * J.dt *
-                                (J.form == 0 ? J.inverseRe : J.vis);
+        double lap = V[q] - (mx * (V[q + st[0]] - 2.0 * V[q] + V[q - st[0]]) * J.ih2[0] +
+                             my * (V[q + st[1]] - 2.0 * V[q] + V[q - st[1]]) * J.ih2[1] +
+                             mz * (V[q + 1] - 2.0 * V[q] + V[q - 1]) * J.ih2[2]) * J.lapS;
         e = fmax(e, fabs(lap - R[q]));
     }
     __shared__ double s[4];
@@ -208,7 +215,52 @@ __global__ void __launch_bounds__(256) k_jacobi_res(FShape f, Metrics M, Bulk B,
     for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = e;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+    if (threadIdx.x == 0) part[lb] = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+}
+
+// One Jacobi sweep G = J(V), as k_jacobi_v, and from the same loads the convergence residual of V
+// itself, as k_jacobi_res: the sweep that follows an iterate and that iterate's residual read the
+// same neighbours.  Both expressions are those of the two kernels, so the iterates and the residuals
+// are theirs.  A logical block covers 1024 consecutive padded entries (four per thread) and stores
+// the maximum residual of its bulk entries in part.
+__global__ void __launch_bounds__(256) k_jacobi_vres(FShape f, Metrics M, Bulk B, const double *__restrict__ V,
+                                                     const double *__restrict__ R, double *__restrict__ G,
+                                                     JacobiCoef J, double *__restrict__ part) {
+    const unsigned n1 = (unsigned)(f.n[1] + 2), n2 = (unsigned)(f.n[2] + 2);
+    const unsigned total = (unsigned)(f.n[0] + 2) * n1 * n2;
+    const unsigned lb = xcd_block();
+    const long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
+    double e = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const unsigned t = lb * 1024u + (unsigned)u * 256u + threadIdx.x;
+        if (t >= total) break;
+        unsigned r = t / n2;
+        int k = (int)(t - r * n2) - 1;
+        unsigned i0 = r / n1;
+        int j = (int)(r - i0 * n1) - 1;
+        int i = (int)i0 - 1;
+        if (i < B.lo[0] || i > B.hi[0] || j < B.lo[1] || j > B.hi[1] || k < B.lo[2] || k > B.hi[2]) {
+            G[t] = 0.0;
+            continue;
+        }
+        const long q = t;
+        const double vxp = V[q + st[0]], vxm = V[q - st[0]], vyp = V[q + st[1]], vym = V[q - st[1]];
+        const double vzp = V[q + 1], vzm = V[q - 1], v0 = V[q], rq = R[q];
+        double mx = metv(M, 0, J.ms[0], 3, i), my = metv(M, 1, J.ms[1], 3, j), mz = metv(M, 2, J.ms[2], 3, k);
+        double a = J.hy2hz2 * mx * (vxp + vxm) + J.hz2hx2 * my * (vyp + vym) + J.hx2hy2 * mz * (vzp + vzm);
+        double d = J.hy2hz2 * mx + J.hz2hx2 * my + J.hx2hy2 * mz;
+        G[q] = (a * J.rA + rq) / (1.0 + d * J.rD);
+        double lap = v0 - (mx * (vxp - 2.0 * v0 + vxm) * J.ih2[0] + my * (vyp - 2.0 * v0 + vym) * J.ih2[1] +
+                           mz * (vzp - 2.0 * v0 + vzm) * J.ih2[2]) * J.lapS;
+        e = fmax(e, fabs(lap - rq));
+    }
+    __shared__ double sm[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) part[lb] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
 }
 
 // maximum of the block maxima, stored as the bit pattern of the double
@@ -263,7 +315,8 @@ __global__ void __launch_bounds__(256) k_build_trhs(FShape fp, FShape fx, FShape
                                                     double *__restrict__ R, double h0, double h1, double h2,
                                                     double kappa, double dt) {
     long total = (long)(fp.n[0] + 2) * (fp.n[1] + 2) * (fp.n[2] + 2);
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    long t = (long)lb * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int k = (int)(t % (fp.n[2] + 2)) - 1;
     long r = t / (fp.n[2] + 2);
@@ -312,7 +365,8 @@ __global__ void __launch_bounds__(256) k_nusselt(FShape fp, FShape fz, Metrics M
                                                  const double *__restrict__ T, double h0, double h1, double h2,
                                                  double *__restrict__ part) {
     long total = (long)fp.n[0] * fp.n[1] * fp.n[2];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    long t = (long)lb * blockDim.x + threadIdx.x;
     double vol = 0.0, uzt = 0.0;
     if (t < total) {
         int k = (int)(t % fp.n[2]);
@@ -335,8 +389,8 @@ __global__ void __launch_bounds__(256) k_nusselt(FShape fp, FShape fz, Metrics M
     if (threadIdx.x == 0) {
         double a = 0, b = 0;
         for (int w = 0; w < 4; w++) { a += sv[w]; b += su[w]; }
-        part[2 * (long)blockIdx.x] = a;
-        part[2 * (long)blockIdx.x + 1] = b;
+        part[2 * (long)lb] = a;
+        part[2 * (long)lb + 1] = b;
     }
 }
 
@@ -348,7 +402,8 @@ __global__ void __launch_bounds__(256) k_divergence(FShape fp, FShape fx, FShape
                                                     const double *__restrict__ Vz, double *__restrict__ out,
                                                     double h0, double h1, double h2, double scale, int scaled) {
     long total = (long)fp.n[0] * fp.n[1] * fp.n[2];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    long t = (long)lb * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int k = (int)(t % fp.n[2]);
     long r = t / fp.n[2];
@@ -367,7 +422,8 @@ __global__ void __launch_bounds__(256) k_divergence(FShape fp, FShape fx, FShape
 __global__ void __launch_bounds__(256) k_project(int c, FShape fc, FShape fp, Metrics M, const double *__restrict__ Pp,
                                                  double *__restrict__ V, double hc, double dt) {
     long total = (long)fc.n[0] * fc.n[1] * fc.n[2];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    long t = (long)lb * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int k = (int)(t % fc.n[2]);
     long r = t / fc.n[2];
@@ -400,7 +456,8 @@ __global__ void __launch_bounds__(256) k_energy(FShape fp, FShape fx, FShape fy,
                                                 const double *__restrict__ Vz, double h0, double h1, double h2,
                                                 double *__restrict__ part) {
     long total = (long)fp.n[0] * fp.n[1] * fp.n[2];
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    long t = (long)lb * blockDim.x + threadIdx.x;
     double vol = 0.0, en = 0.0;
     if (t < total) {
         int k = (int)(t % fp.n[2]);
@@ -425,8 +482,8 @@ __global__ void __launch_bounds__(256) k_energy(FShape fp, FShape fx, FShape fy,
     if (threadIdx.x == 0) {
         double a = 0, b = 0;
         for (int w = 0; w < 4; w++) { a += sv[w]; b += se[w]; }
-        part[2 * (long)blockIdx.x] = a;
-        part[2 * (long)blockIdx.x + 1] = b;
+        part[2 * (long)lb] = a;
+        part[2 * (long)lb + 1] = b;
     }
 }
 
@@ -535,10 +592,27 @@ void ts_nusselt(const FShape &fp, const FShape &fz, const Metrics &M, const doub
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, part, n, out);
 }
 
+// the grid-uniform quotients of hydro_d3.cc:375-410 (form 0: 1/Re; form 1: the diffusivity vis,
+// scalar_d3.cc:373-531), formed once on the host
+static JacobiCoef with_quotients(JacobiCoef J) {
+    if (J.form == 0) {
+        J.rA = J.dt / (J.hx2hy2hz2 * 2.0 * J.Re);
+        J.rD = J.dt / (J.Re * J.hx2hy2hz2);
+    } else {
+        J.rA = J.dt * J.vis / (J.hx2hy2hz2 * 2.0);
+        J.rD = J.dt * J.vis / J.hx2hy2hz2;
+    }
+    J.ih2[0] = 1.0 / (J.hx * J.hx);
+    J.ih2[1] = 1.0 / (J.hy * J.hy);
+    J.ih2[2] = 1.0 / (J.hz * J.hz);
+    J.lapS = 0.5 * J.dt * (J.form == 0 ? J.inverseRe : J.vis);
+    return J;
+}
+
 void ts_jacobi(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
                const JacobiCoef &J, hipStream_t s) {
     long n = (long)(f.n[0] + 2) * (f.n[1] + 2) * (f.n[2] + 2);
-    hipLaunchKernelGGL(k_jacobi_v, dim3(nb(n)), dim3(256), 0, s, f, M, B, V, R, G, J);
+    hipLaunchKernelGGL(k_jacobi_v, dim3(nb(n)), dim3(256), 0, s, f, M, B, V, R, G, with_quotients(J));
 }
 
 int ts_jacobi_res_parts() { return 1024; }
@@ -549,7 +623,18 @@ void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const doubl
     unsigned g = nb(n);
     if (g > (unsigned)ts_jacobi_res_parts()) g = (unsigned)ts_jacobi_res_parts();
     if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_jacobi_res, dim3(g), dim3(256), 0, s, f, M, B, V, R, J, part);
+    hipLaunchKernelGGL(k_jacobi_res, dim3(g), dim3(256), 0, s, f, M, B, V, R, with_quotients(J), part);
+    hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(256), 0, s, part, (int)g, out);
+}
+
+long ts_jacobi_vres_parts(const FShape &f) {
+    return ((long)(f.n[0] + 2) * (f.n[1] + 2) * (f.n[2] + 2) + 1023) / 1024;
+}
+
+void ts_jacobi_vres(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
+                    const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s) {
+    unsigned g = (unsigned)ts_jacobi_vres_parts(f);
+    hipLaunchKernelGGL(k_jacobi_vres, dim3(g), dim3(256), 0, s, f, M, B, V, R, G, with_quotients(J), part);
     hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(256), 0, s, part, (int)g, out);
 }
 
