@@ -77,18 +77,20 @@ __device__ inline bool geo_slot(const Geo &g, unsigned b, unsigned &ip, unsigned
 // ------------------------------------------------------------------------------------------------
 // ghost composition (imposeBC order x, y, z; z outermost)
 // ------------------------------------------------------------------------------------------------
+// Where ghost (ip, jp, kp) takes its value in the composition: 0 it is not written, 1 it is
+// sign * a[src] (src a core node), 2 it is the constant cval.
 template <typename T>
-__device__ inline bool ghost_resolve(const Lay &L, const T *a, int ip, int jp, int kp, const Bc &bc, T &out) {
+__device__ inline int ghost_source(const Lay &L, int ip, int jp, int kp, const Bc &bc, long &src, T &sign, T &cval) {
     int n[3] = {L.nx, L.ny, L.nz};
     int q[3] = {ip, jp, kp};
-    T sign = T(1);
+    T sg = T(1);
     bool moved = false;
     for (int d = 2; d >= 0; --d) {
         int g = q[d];
         if (g != 0 && g != n[d] + 1) continue;
         int r = bc.rule[d];
         if (r == G_KEEP) continue;
-        if (r == G_ZERO) { out = T(0); return true; }
+        if (r == G_ZERO) { cval = T(0); return 2; }
         if (r == G_WALL) {
             const double *w = bc.wall[2 * d + (g == 0 ? 0 : 1)];
             T v = T(0);
@@ -98,15 +100,26 @@ __device__ inline bool ghost_resolve(const Lay &L, const T *a, int ip, int jp, i
                                  : (long)q[0] * (L.ny + 2) + q[1];
                 v = (T)w[wi];
             }
-            out = sign * v;
-            return true;
+            cval = sg * v;
+            return 2;
         }
         q[d] = ghost_src(r, g, n[d]);
-        if (r == G_ANTI) sign = -sign;
+        if (r == G_ANTI) sg = -sg;
         moved = true;
     }
-    if (!moved) return false;
-    out = sign * a[lay_idx(L, q[0], q[1], q[2])];
+    if (!moved) return 0;
+    src = lay_idx(L, q[0], q[1], q[2]);
+    sign = sg;
+    return 1;
+}
+
+template <typename T>
+__device__ inline bool ghost_resolve(const Lay &L, const T *a, int ip, int jp, int kp, const Bc &bc, T &out) {
+    long src = 0;
+    T sign = T(1), cval = T(0);
+    int kind = ghost_source<T>(L, ip, jp, kp, bc, src, sign, cval);
+    if (kind == 0) return false;
+    out = kind == 1 ? sign * a[src] : cval;
     return true;
 }
 
@@ -1189,6 +1202,98 @@ __device__ inline double wv_max_res(const Lay &L, const T *u, const T *f, const 
     return wave_max(m);
 }
 
+// The same coarsest-level iteration for a level of at most 64 core nodes and 256 padded entries:
+// each lane keeps one node's neighbour indices, coefficients and right-hand side in registers, and
+// up to four ghost entries with where they take their value, so an iteration is a few LDS reads
+// per lane instead of the index arithmetic and ghost composition of every pass.  The expressions
+// are those of wv_sweep and wv_max_res, and the order of the reference loop is kept: the ghosts
+// whose source has colour c are refilled after the colour-c half sweep, which is where the next
+// imposeBC would first change them, and the colour-1 ones only after the residual, which the
+// reference evaluates before its next imposeBC (poisson_d3.cc:190-247).  Returns the iteration count.
+template <typename T>
+__device__ inline int wv_coarsest_small(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz,
+                                        const Bc &bc, double tol, int max_count, int lane) {
+    const int nc = L.nx * L.ny * L.nz;
+    const bool own = lane < nc;
+    long me = 0, nb[6] = {0, 0, 0, 0, 0, 0};
+    T co[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, dg = T(1), fm = T(0);
+    int col = 2;
+    if (own) {
+        int k = lane % L.nz, r = lane / L.nz;
+        int j = r % L.ny, i = r / L.ny;
+        int ip = i + 1, jp = j + 1, kp = k + 1;
+        me = lay_idx(L, ip, jp, kp);
+        nb[0] = lay_idx(L, ip + 1, jp, kp);
+        nb[1] = lay_idx(L, ip - 1, jp, kp);
+        nb[2] = lay_idx(L, ip, jp + 1, kp);
+        nb[3] = lay_idx(L, ip, jp - 1, kp);
+        nb[4] = lay_idx(L, ip, jp, kp + 1);
+        nb[5] = lay_idx(L, ip, jp, kp - 1);
+        const T *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
+        co[0] = X[0]; co[1] = X[1]; co[2] = Y[0]; co[3] = Y[1]; co[4] = Z[0]; co[5] = Z[1];
+        dg = diag3<T>(X, Y, Z);
+        fm = f[me];
+        col = (i + j + k) & 1;
+    }
+    const int NX = L.nx + 2, NY = L.ny + 2, NZ = L.nz + 2;
+    const int np = NX * NY * NZ;
+    long gd[4] = {0, 0, 0, 0}, gs[4] = {0, 0, 0, 0};
+    T gsg[4] = {T(1), T(1), T(1), T(1)}, gc[4] = {T(0), T(0), T(0), T(0)};
+    int gk[4] = {0, 0, 0, 0}, gcol[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        int p = lane + 64 * q;
+        if (p >= np) continue;
+        int kp = p % NZ, r = p / NZ;
+        int jp = r % NY, ip = r / NY;
+        if (!(ip == 0 || ip == NX - 1 || jp == 0 || jp == NY - 1 || kp == 0 || kp == NZ - 1)) continue;
+        gk[q] = ghost_source<T>(L, ip, jp, kp, bc, gs[q], gsg[q], gc[q]);
+        gd[q] = lay_idx(L, ip, jp, kp);
+        if (gk[q] == 1) {
+            // colour of the source node (lay_idx's row carries it)
+            long row = gs[q] / L.khs;
+            gcol[q] = (int)(row & 1);
+        }
+    }
+    // colour c < 2: the ghosts whose source has colour c; c == 2: every ghost
+    auto fill = [&](int c) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (gk[q] == 1 && (c == 2 || gcol[q] == c)) u[gd[q]] = gsg[q] * u[gs[q]];
+            else if (gk[q] == 2) u[gd[q]] = gc[q];
+        }
+    };
+    fill(2);
+    wave_sync();
+    int it = 0;
+    while (true) {
+        if (col == 0)
+            u[me] = (co[0] * u[nb[0]] + co[1] * u[nb[1]] + co[2] * u[nb[2]] + co[3] * u[nb[3]] + co[4] * u[nb[4]] +
+                     co[5] * u[nb[5]] - fm) / dg;
+        wave_sync();
+        fill(0);
+        wave_sync();
+        if (col == 1)
+            u[me] = (co[0] * u[nb[0]] + co[1] * u[nb[1]] + co[2] * u[nb[2]] + co[3] * u[nb[3]] + co[4] * u[nb[4]] +
+                     co[5] * u[nb[5]] - fm) / dg;
+        wave_sync();
+        double m = -1.0e-10;
+        if (own) {
+            T c = u[me];
+            T lu = co[0] * u[nb[0]] + co[1] * u[nb[1]] + co[2] * u[nb[2]] + co[3] * u[nb[3]] + co[4] * u[nb[4]] +
+                   co[5] * u[nb[5]] - dg * c;
+            m = fmax(m, fabs((double)fm - (double)lu));
+        }
+        const double gm = wave_max(m);
+        fill(1);
+        wave_sync();
+        if (gm < tol) break;
+        it += 1;
+        if (it > max_count) break;
+    }
+    return it;
+}
+
 // u = P f on the core nodes of the coarsest level (P transposed: pt[j * nc + i] = P[i][j]), node
 // order (i, j, k) with k fastest; ghosts are left to the caller.  FP64 accumulation.
 template <typename T>
@@ -1260,7 +1365,12 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
             if (threadIdx.x == 0) iters[0] = 0;
         } else {
         if (D.project) blk_project<T>(L, FF(q), D.wc[0], D.wc[1], D.wc[2], D.wcsum, red);
-        if (threadIdx.x < 64) {
+        const bool small = L.nx * L.ny * L.nz <= 64 && (L.nx + 2) * (L.ny + 2) * (L.nz + 2) <= 256;
+        if (small && threadIdx.x < 64) {
+            const int it = wv_coarsest_small<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.bc, D.tol, D.max_count,
+                                                (int)threadIdx.x);
+            if (threadIdx.x == 0) iters[0] = it;
+        } else if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
             int it = 0;
             while (true) {
