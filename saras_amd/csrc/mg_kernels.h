@@ -28,6 +28,8 @@ template <typename T>
 void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int core, hipStream_t s);
 template <typename TS, typename TD> void launch_pack(const Lay &L, const TS *ref, TD *dst, int core_only, hipStream_t s);
 template <typename TS, typename TD> void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s);
+// every entry of the split layout: core values from the padded array, 0 elsewhere
+template <typename TS, typename TD> void launch_pack_full(const Lay &L, const TS *ref, TD *dst, hipStream_t s);
 int norm_parts(const Lay &L);
 // out[0] = sum_core w_i w_j w_k f, out[1] = sum over every padded node of f
 template <typename T>

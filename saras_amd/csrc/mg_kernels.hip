@@ -399,6 +399,25 @@ __global__ void __launch_bounds__(256) k_pack(Lay L, const TS *__restrict__ ref,
     dst[lay_idx(L, ip, jp, kp)] = (TD)ref[t];
 }
 
+// Every entry of the split layout in one pass: the core node's value from the padded reference
+// array, 0 on ghosts and unused slots (the state of a zero fill followed by a core-only pack).
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) k_pack_full(Lay L, const TS *__restrict__ ref, TD *__restrict__ dst) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= lay_size(L)) return;
+    long row = t / L.khs;
+    int h = (int)(t - row * L.khs);
+    int c = (int)(row & 1);
+    long col = row >> 1;
+    int jp = (int)(col % (L.ny + 2));
+    int ip = (int)(col / (L.ny + 2));
+    int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+    TD v = TD(0);
+    if (ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz)
+        v = (TD)ref[((long)ip * (L.ny + 2) + jp) * (L.nz + 2) + kp];
+    dst[t] = v;
+}
+
 template <typename TS, typename TD>
 __global__ void __launch_bounds__(256) k_unpack(Lay L, const TS *__restrict__ src, TD *__restrict__ ref) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1505,6 +1524,11 @@ void launch_pack(const Lay &L, const TS *ref, TD *dst, int core_only, hipStream_
 }
 
 template <typename TS, typename TD>
+void launch_pack_full(const Lay &L, const TS *ref, TD *dst, hipStream_t s) {
+    hipLaunchKernelGGL((k_pack_full<TS, TD>), dim3(nblk(lay_size(L))), dim3(256), 0, s, L, ref, dst);
+}
+
+template <typename TS, typename TD>
 void launch_unpack(const Lay &L, const TS *src, TD *ref, hipStream_t s) {
     long n = (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
     hipLaunchKernelGGL((k_unpack<TS, TD>), dim3(nblk(n)), dim3(256), 0, s, L, src, ref);
@@ -1685,6 +1709,7 @@ template void launch_prolong<float, float>(const Lay &, const Lay &, const float
 template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t, int);
 template void launch_pack<double, double>(const Lay &, const double *, double *, int, hipStream_t);
 template void launch_pack<double, float>(const Lay &, const double *, float *, int, hipStream_t);
+template void launch_pack_full<double, double>(const Lay &, const double *, double *, hipStream_t);
 template void launch_unpack<double, double>(const Lay &, const double *, double *, hipStream_t);
 template void launch_unpack<float, double>(const Lay &, const float *, double *, hipStream_t);
 

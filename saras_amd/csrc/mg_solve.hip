@@ -67,11 +67,8 @@ static int stage_alloc(sr_mg *m) {
 static void load_level0(sr_mg *m, const double *drhs, const double *dp) {
     Level &V = m->lv[0];
     hipStream_t s = m->stream;
-    long sz = lay_size(V.L);
-    launch_fill<double>(V.u, sz, 0.0, s);
-    launch_fill<double>(V.f, sz, 0.0, s);
-    launch_pack<double, double>(V.L, drhs, V.f, 1, s);
-    launch_pack<double, double>(V.L, dp, V.u, 1, s);
+    launch_pack_full<double, double>(V.L, drhs, V.f, s);
+    launch_pack_full<double, double>(V.L, dp, V.u, s);
     bool proj = m->d.semantics == SR_SEM_PROJECTED && is_singular(m);
     if (proj) {
         launch_rhs_sums<double>(V.L, V.f, m->wvec[0], m->wvec[1], m->wvec[2], m->part, m->dres, s);
