@@ -313,6 +313,7 @@ def rbc_bench(args):
                "cycles_mean": float(np.mean([r.mg_cycles for r in rec_l])),
                "cycles_per_step": [int(r.mg_cycles) for r in rec_l],
                "guess_ms_mean": float(np.mean([r.guess_ms for r in rec_l])),
+               "guess_predict_ms_mean": float(np.mean([r.guess_predict_ms for r in rec_l])),
                "choices": {name: int(sum(1 for r in rec_l if r.guess_choice == c))
                            for c, name in ((0, "warm"), (1, "extrapolation"), (2, "learned"), (3, "linear"))},
                "rel_guess_median": float(np.median([r.rel_guess for r in rec_l])),

@@ -79,6 +79,7 @@ typedef struct sr_ts_stats {
                                     /* residual (exact when the network ran, else its fit's estimate), */
                                     /* -1 without one                                              */
     double guess_rel_learned;       /* sr_guess_stats.rel_learned, -1 when the network did not run */
+    double guess_predict_ms;        /* sr_guess_stats.predict_ms: the part of guess_ms before the solve */
 } sr_ts_stats;
 
 /* hydro_d3::hydro_d3 (src/solvers/hydro/hydro_d3.cc:63-119) without the initial condition */

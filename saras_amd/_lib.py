@@ -67,7 +67,8 @@ class SrTsStats(C.Structure):
                 ("solve_ms", C.c_double), ("guess_ms", C.c_double), ("guess_choice", C.c_int),
                 ("rel_guess", C.c_double), ("train_loss", C.c_double), ("sweep0_ms", C.c_double),
                 ("mg_converged", C.c_int), ("mg_residual", C.c_double),
-                ("guess_rel_linear", C.c_double), ("guess_rel_learned", C.c_double)]
+                ("guess_rel_linear", C.c_double), ("guess_rel_learned", C.c_double),
+                ("guess_predict_ms", C.c_double)]
 
 
 SR_GUESS_LAYERS, SR_GUESS_IN_CH = 12, 8

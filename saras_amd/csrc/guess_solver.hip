@@ -288,19 +288,23 @@ __device__ __forceinline__ void core_of(const FineDims &f, unsigned t, int &i, i
     i = (int)i0;
 }
 
-// per-block sums of width W (W <= 16) through LDS: part[b * W + c]
+// per-block sums of width W (W <= 16) of a 256-thread block: each wave adds its lanes with
+// shuffles, then one thread per column adds the four wave sums in a fixed order; part[b * W + c]
 template <int W>
 __device__ inline void block_sums(double (&v)[W], double *part) {
-    __shared__ double red[256];
+    __shared__ double red[W][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
     for (int c = 0; c < W; c++) {
-        red[threadIdx.x] = v[c];
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(long)blockIdx.x * W + c] = red[0];
-        __syncthreads();
+        double x = v[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[c][wv] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < W) {
+        const int c = threadIdx.x;
+        part[(long)blockIdx.x * W + c] = (red[c][0] + red[c][1]) + (red[c][2] + red[c][3]);
     }
 }
 

@@ -566,7 +566,7 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
     // mgRHS = div V / dt (hydro_d3.cc:336-340); pads stay 0
     ts_divergence(t->fp, t->fv, t->M, Vc, t->mgRHS, t->h, 1.0 / dt, 1, t->s);
     HIPCHK(hipGetLastError());
-    double guess_ms = 0.0, rel_guess = -1.0, loss = -1.0, rel_linear = -1.0, rel_learned = -1.0;
+    double guess_ms = 0.0, rel_guess = -1.0, loss = -1.0, rel_linear = -1.0, rel_learned = -1.0, predict_ms = 0.0;
     int choice = -1;
     if (t->guess && t->nsolved >= 1) {
         HIPCHK(hipStreamSynchronize(t->s));
@@ -574,6 +574,7 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
         RETCHK(sr_guess_predict(t->guess, t->mg, t->Pp1, t->nsolved >= 2 ? t->Pp2 : nullptr, t->mgRHS, t->Pp, &gs));
         choice = gs.choice;
         guess_ms = gs.predict_ms;
+        predict_ms = gs.predict_ms;
         rel_linear = gs.rel_linear;
         rel_learned = gs.rel_learned;
         rel_guess = choice == 3 ? gs.rel_linear : choice == 2 ? gs.rel_learned : choice == 1 ? gs.rel_extrap : gs.rel_warm;
@@ -634,6 +635,7 @@ extern "C" int sr_ts_time_advance(sr_ts *t, sr_ts_stats *st) {
         st->mg_residual = ms.residual;
         st->guess_rel_linear = rel_linear;
         st->guess_rel_learned = rel_learned;
+        st->guess_predict_ms = predict_ms;
         st->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
     return SR_OK;
