@@ -80,7 +80,7 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
     if (g->d.beta2 <= 0.f) g->d.beta2 = 0.999f;
     if (g->d.eps <= 0.f) g->d.eps = 1e-8f;
     HIPCHK(hipStreamCreateWithFlags(&g->s, hipStreamNonBlocking));
-    long partmax = 0;
+    long partmax = 0, bpartmax = 0;
     for (int l = 0; l < SR_GUESS_LAYERS; l++) {
         const LayerSpec &L = kLayers[l];
         RETCHK(dalloc(g, &g->W[l], wsize(l)));
@@ -94,6 +94,7 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
         RETCHK(dalloc(g, &g->pf[l], conv_pack_elems(L.cin, L.cout)));
         RETCHK(dalloc(g, &g->pb[l], conv_pack_elems(L.cout, L.cin)));
         partmax = std::max(partmax, wgrad_part_elems(L.cin, L.cout, g->lv[L.level]));
+        bpartmax = std::max(bpartmax, (long)wgrad_slots(L.cin, L.cout, g->lv[L.level]) * L.cout);
     }
     for (int t = 0; t < 18; t++) RETCHK(dalloc(g, &g->T[t], tensor_elems(g, t)));
     const long n0 = g->lv[0].nvox(), n1 = g->lv[1].nvox(), n2 = g->lv[2].nvox();
@@ -107,7 +108,7 @@ static int create_impl(const sr_guess_desc *desc, sr_guess *g) {
     RETCHK(dalloc(g, &g->tM, tmax));
     RETCHK(dalloc(g, &g->dy16, n0 * 16));
     RETCHK(dalloc(g, &g->part, partmax));
-    RETCHK(dalloc(g, &g->bpart, 512L * 64));
+    RETCHK(dalloc(g, &g->bpart, bpartmax));
     RETCHK(dalloc(g, &g->stage, n0 * SR_GUESS_IN_CH));
     g->fine_pad = (long)(g->fine[0] + 2) * (g->fine[1] + 2) * (g->fine[2] + 2);
     RETCHK(dalloc(g, &g->extrap, g->fine_pad));
@@ -232,9 +233,9 @@ static int conv_adj(sr_guess *g, int l, const bf16_t *dY, float *dX) {
 
 static int wgrad(sr_guess *g, int l, const bf16_t *dY, const bf16_t *X) {
     const LayerSpec &L = kLayers[l];
-    if (launch_wgrad(L.cin, L.cout, g->lv[L.level], dY, X, g->part, g->dW[l], g->s) != 0)
+    // the bias gradient comes out of the same pass over dY
+    if (launch_wgrad(L.cin, L.cout, g->lv[L.level], dY, X, g->part, g->dW[l], g->bpart, g->db[l], g->s) != 0)
         return set_err(SR_ERR_ARG, "no weight-gradient kernel for this channel pair");
-    launch_bias_grad(L.cout, g->lv[L.level], dY, g->bpart, g->db[l], g->s);
     return SR_OK;
 }
 

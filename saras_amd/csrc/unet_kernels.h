@@ -35,10 +35,10 @@ int launch_conv(int cin, int cout, const NetGrid &g, const bf16_t *in, const bf1
 // weight gradient: partials per slot, then the fixed-order sum into dW (27, COUT, CIN) fp32
 int wgrad_slots(int cin, int cout, const NetGrid &g);
 long wgrad_part_elems(int cin, int cout, const NetGrid &g);
+// with bpart (wgrad_slots x cout floats) the bias gradient db (cout) of the same output gradient
+// comes from the same pass: the blocks of input block 0 add the staged gradient tiles per channel
 int launch_wgrad(int cin, int cout, const NetGrid &g, const bf16_t *dy, const bf16_t *x, float *part, float *dW,
-                 hipStream_t s);
-int bias_slots(const NetGrid &g);
-void launch_bias_grad(int cout, const NetGrid &g, const bf16_t *dy, float *part, float *db, hipStream_t s);
+                 float *bpart, float *db, hipStream_t s);
 // transfer operators on channels-last tensors (C a multiple of 4)
 void launch_restrict_bf16(const NetGrid &fine, const NetGrid &coarse, int C, const bf16_t *in, bf16_t *out,
                           hipStream_t s);

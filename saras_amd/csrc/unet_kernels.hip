@@ -159,76 +159,138 @@ __global__ void __launch_bounds__(256) k_conv(NetGrid g, const bf16_t *__restric
 }
 
 // Weight gradient of one (16-channel output block, 16-channel input block) pair:
-// D[co][ci] += dY^T[co][voxel] X[voxel + tap][ci], summed over the tiles of one slot.  Wave w owns
-// the taps w, w + 4, ..., so each tap's accumulator lives in exactly one wave; the A fragment
-// (output gradient) of a 32-voxel K-step is shared by the wave's taps.
+// D[co][ci] += dY^T[co][voxel] X[voxel + tap][ci], summed over the tiles of one slot.  The
+// contraction runs over voxels, which are strided in the channels-last tensors, so both tiles are
+// staged in LDS transposed -- channel-major with the z voxels contiguous -- and the input halo as
+// three copies shifted by dz = -1, 0, 1: every MFMA operand (8 consecutive z voxels of one channel)
+// is then one aligned 16-byte LDS read.  Wave w owns the taps w, w + 4, ..., so each tap's
+// accumulator lives in exactly one wave; the A fragment (output gradient) of a 32-voxel K-step is
+// shared by the wave's taps.  Row strides: 264 and 584 elements (132 and 292 dwords), so the 16
+// lanes of a fragment read hit disjoint banks.
+constexpr int WG_GST = 264;   // gT row: 256 voxels (16 lines x 16 z) + 8
+constexpr int WG_XST = 584;   // xT channel: 36 halo lines (6 x 6) x 16 z + 8
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_wgrad(NetGrid g, const bf16_t *__restrict__ dy, const bf16_t *__restrict__ x,
-                                               float *__restrict__ part, int nslots) {
+                                               float *__restrict__ part, int nslots, float *__restrict__ bpart) {
     constexpr int NIB = (CIN + 15) / 16, NPAIR = (COUT / 16) * NIB;
     const int pair = blockIdx.y, cb = pair / NIB, ib = pair % NIB;
     const int slot = blockIdx.x;
-    __shared__ __attribute__((aligned(16))) bf16_t xs[648 * 24];
-    __shared__ __attribute__((aligned(16))) bf16_t gs[256 * 24];
+    __shared__ __attribute__((aligned(16))) bf16_t gT[16 * WG_GST];
+    __shared__ __attribute__((aligned(16))) bf16_t xT[3 * 16 * WG_XST];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m = lane & 15, gq = lane >> 4;
     f32x4 acc[7];
 #pragma unroll
     for (int j = 0; j < 7; j++) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const long ntiles = g.ntiles();
-    for (long tile = slot; tile < ntiles; tile += nslots) {
-        __syncthreads();
+    // The next tile's halo and gradient items are loaded into registers while the current tile
+    // computes (each thread owns items tid + 256 k: at most 6 halo items and one gradient pair).
+    u32x4 xr[6], ga, gb;
+    auto load = [&](long tile) {
         const int tzi = (int)(tile % g.tz), rr = (int)(tile / g.tz), tyi = rr % g.ty, txi = rr / g.ty;
         const int x0 = txi * 4, y0 = tyi * 4, z0 = tzi * 16;
-        for (int idx = threadIdx.x; idx < 648 * 2; idx += 256) {
-            int v = idx >> 1, p = idx & 1;
-            int hz = v % 18, hy = (v / 18) % 6, hx = v / 108;
-            int xx = x0 + hx - 1, yv = y0 + hy - 1, zz = z0 + hz - 1;
-            u32x4 val = {0u, 0u, 0u, 0u};
-            if (xx >= 0 && xx < g.nx && yv >= 0 && yv < g.ny && zz >= 0 && zz < g.nz && ib * 16 + p * 8 < CIN)
-                val = *(const u32x4 *)(x + (((long)xx * g.ny + yv) * g.nz + zz) * CIN + ib * 16 + p * 8);
-            *(u32x4 *)(xs + v * 24 + p * 8) = val;
-        }
-        for (int idx = threadIdx.x; idx < 256 * 2; idx += 256) {
-            int v = idx >> 1, p = idx & 1;
-            int lz = v % 16, ly = (v / 16) % 4, lx = v / 64;
-            int xx = x0 + lx, yv = y0 + ly, zz = z0 + lz;
-            u32x4 val = {0u, 0u, 0u, 0u};
-            if (xx < g.nx && yv < g.ny && zz < g.nz)
-                val = *(const u32x4 *)(dy + (((long)xx * g.ny + yv) * g.nz + zz) * COUT + cb * 16 + p * 8);
-            *(u32x4 *)(gs + v * 24 + p * 8) = val;
-        }
-        __syncthreads();
-        for (int q = 0; q < 8; q++) {
-            // K-step q: voxels kv = 8 gq + i of lines 2q and 2q + 1 of the tile
-            unsigned short av[8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int kv = 8 * gq + i, L = 2 * q + (kv >> 4), zz = kv & 15;
-                av[i] = gs[(L * 16 + zz) * 24 + m];
+        for (int k = 0; k < 6; k++) {
+            const int idx = threadIdx.x + 256 * k;
+            xr[k] = (u32x4){0u, 0u, 0u, 0u};
+            if (idx < 648 * 2) {
+                const int v = idx >> 1, p = idx & 1;
+                const int hz = v % 18, hl = v / 18, hy = hl % 6, hx = hl / 6;
+                const int xx = x0 + hx - 1, yv = y0 + hy - 1, zz = z0 + hz - 1;
+                if (xx >= 0 && xx < g.nx && yv >= 0 && yv < g.ny && zz >= 0 && zz < g.nz && ib * 16 + p * 8 < CIN)
+                    xr[k] = *(const u32x4 *)(x + (((long)xx * g.ny + yv) * g.nz + zz) * CIN + ib * 16 + p * 8);
             }
-            u32x4 ua = {(unsigned)av[0] | ((unsigned)av[1] << 16), (unsigned)av[2] | ((unsigned)av[3] << 16),
-                        (unsigned)av[4] | ((unsigned)av[5] << 16), (unsigned)av[6] | ((unsigned)av[7] << 16)};
-            const bf16x8 A = __builtin_bit_cast(bf16x8, ua);
+        }
+        {
+            const int idx = threadIdx.x;
+            const int vp = idx >> 1, p = idx & 1;
+            const int lz = (vp % 8) * 2, L = vp / 8, ly = L & 3, lx = L >> 2;
+            const int xx = x0 + lx, yv = y0 + ly, zz = z0 + lz;
+            ga = (u32x4){0u, 0u, 0u, 0u};
+            gb = (u32x4){0u, 0u, 0u, 0u};
+            if (xx < g.nx && yv < g.ny) {
+                const long base = (((long)xx * g.ny + yv) * g.nz + zz) * COUT + cb * 16 + p * 8;
+                if (zz < g.nz) ga = *(const u32x4 *)(dy + base);
+                if (zz + 1 < g.nz) gb = *(const u32x4 *)(dy + base + COUT);
+            }
+        }
+    };
+    // input halo (6 x 6 x 18 voxels, 8 channels per item) into the three shifted copies: copy c
+    // holds halo z = z' + c at z' = 0..15; the gradient tile (4 x 4 x 16 voxels) as (z, z + 1) pairs
+    auto store = [&]() {
+        unsigned short *xs16 = (unsigned short *)xT;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const int idx = threadIdx.x + 256 * k;
+            if (idx >= 648 * 2) continue;
+            const int v = idx >> 1, p = idx & 1;
+            const int hz = v % 18, hl = v / 18;
+            const u32x4 val = xr[k];
+            const unsigned short e[8] = {(unsigned short)(val[0] & 0xffffu), (unsigned short)(val[0] >> 16),
+                                         (unsigned short)(val[1] & 0xffffu), (unsigned short)(val[1] >> 16),
+                                         (unsigned short)(val[2] & 0xffffu), (unsigned short)(val[2] >> 16),
+                                         (unsigned short)(val[3] & 0xffffu), (unsigned short)(val[3] >> 16)};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int zp = hz - c;
+                if (zp < 0 || zp > 15) continue;
+#pragma unroll
+                for (int j = 0; j < 8; j++) xs16[(c * 16 + p * 8 + j) * WG_XST + hl * 16 + zp] = e[j];
+            }
+        }
+        {
+            const int idx = threadIdx.x;
+            const int vp = idx >> 1, p = idx & 1;
+            const int lz = (vp % 8) * 2, L = vp / 8;
+            unsigned int *gs32 = (unsigned int *)gT;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                // channels 2j and 2j + 1: (z, z + 1) pairs
+                const unsigned lo0 = ga[j] & 0xffffu, hi0 = ga[j] >> 16, lo1 = gb[j] & 0xffffu, hi1 = gb[j] >> 16;
+                gs32[((p * 8 + 2 * j) * WG_GST + L * 16 + lz) >> 1] = lo0 | (lo1 << 16);
+                gs32[((p * 8 + 2 * j + 1) * WG_GST + L * 16 + lz) >> 1] = hi0 | (hi1 << 16);
+            }
+        }
+    };
+    // bias gradient of the output block: the blocks of input block 0 also add the staged gradient
+    // tile per channel (channel threadIdx / 16, voxels threadIdx % 16 + 16 k, then the 16 lanes)
+    const bool bias = bpart != nullptr && ib == 0;
+    const int bco = threadIdx.x >> 4, bsub = threadIdx.x & 15;
+    float bacc = 0.f;
+    if (slot < ntiles) load(slot);
+    for (long tile = slot; tile < ntiles; tile += nslots) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (tile + nslots < ntiles) load(tile + nslots);
+        if (bias) {
+            const unsigned short *g16 = (const unsigned short *)gT;
+            float t = 0.f;
+#pragma unroll
+            for (int k = 0; k < 16; k++) t += __uint_as_float((unsigned)g16[bco * WG_GST + bsub + 16 * k] << 16);
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            bacc += t;
+        }
+#pragma unroll 2
+        for (int q = 0; q < 8; q++) {
+            // K-step q: voxels 8 gq + i (i = 0..7) = z 8 (gq & 1) + i of line 2 q + (gq >> 1)
+            const int L = 2 * q + (gq >> 1), zo = 8 * (gq & 1);
+            const bf16x8 A = *(const bf16x8 *)(gT + m * WG_GST + L * 16 + zo);
+            const int lx = L >> 2, ly = L & 3;
 #pragma unroll
             for (int j = 0; j < 7; j++) {
                 const int tap = w + 4 * j;
                 if (tap < 27) {
-                    const int dx = tap / 9 - 1, dyy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-                    unsigned short bv[8];
-#pragma unroll
-                    for (int i = 0; i < 8; i++) {
-                        const int kv = 8 * gq + i, L = 2 * q + (kv >> 4), zz = kv & 15;
-                        const int lx = L >> 2, ly = L & 3;
-                        bv[i] = xs[(((lx + 1 + dx) * 6 + (ly + 1 + dyy)) * 18 + (zz + 1 + dz)) * 24 + m];
-                    }
-                    u32x4 ub = {(unsigned)bv[0] | ((unsigned)bv[1] << 16), (unsigned)bv[2] | ((unsigned)bv[3] << 16),
-                                (unsigned)bv[4] | ((unsigned)bv[5] << 16), (unsigned)bv[6] | ((unsigned)bv[7] << 16)};
-                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, __builtin_bit_cast(bf16x8, ub), acc[j], 0, 0, 0);
+                    const int dx = tap / 9 - 1, dyy = (tap / 3) % 3 - 1, c = tap % 3;
+                    const int hl = (lx + 1 + dx) * 6 + (ly + 1 + dyy);
+                    const bf16x8 B = *(const bf16x8 *)(xT + (c * 16 + m) * WG_XST + hl * 16 + zo);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[j], 0, 0, 0);
                 }
             }
         }
     }
+    if (bias && bsub == 0) bpart[(long)slot * COUT + cb * 16 + bco] = bacc;
     // D[co = 4 gq + r][ci = m] of tap w + 4 j
 #pragma unroll
     for (int j = 0; j < 7; j++) {
@@ -267,25 +329,6 @@ __global__ void __launch_bounds__(256) k_wgrad_final(const float *__restrict__ p
         float t = 0.f;
         for (int q = 0; q < 16; q++) t += red[q][threadIdx.x];
         dW[e] = t;
-    }
-}
-
-__global__ void __launch_bounds__(256) k_bias_part(NetGrid g, int cout, const bf16_t *__restrict__ dy, int nslots,
-                                                   float *__restrict__ part) {
-    __shared__ float red[256];
-    const int per = 256 / cout;           // lanes per channel
-    const int ch = threadIdx.x % cout, sub = threadIdx.x / cout;
-    float s = 0.f;
-    if (sub < per) {
-        const long nv = g.nvox();
-        for (long v = (long)blockIdx.x * per + sub; v < nv; v += (long)nslots * per) s += bf2f(dy[v * cout + ch]);
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < cout) {
-        float t = 0.f;
-        for (int k = 0; k < per; k++) t += red[k * cout + threadIdx.x];
-        part[(long)blockIdx.x * cout + threadIdx.x] = t;
     }
 }
 
@@ -474,14 +517,16 @@ long wgrad_part_elems(int cin, int cout, const NetGrid &g) {
 
 #define WGRAD_CASE(CI, CO)                                                                                 \
     if (cin == CI && cout == CO) {                                                                         \
-        hipLaunchKernelGGL((k_wgrad<CI, CO>), dim3(nslots, npair), dim3(256), 0, s, g, dy, x, part, nslots); \
+        hipLaunchKernelGGL((k_wgrad<CI, CO>), dim3(nslots, npair), dim3(256), 0, s, g, dy, x, part, nslots, bpart); \
         hipLaunchKernelGGL(k_wgrad_final, dim3((unsigned)((27L * cin * cout + 15) / 16)), dim3(256), 0, s, part, cin, \
                            cout, nslots, dW);                                                              \
+        if (bpart) hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(256), 0, s, bpart, cout, nslots, db);   \
         return 0;                                                                                          \
     }
 
 int launch_wgrad(int cin, int cout, const NetGrid &g, const bf16_t *dy, const bf16_t *x, float *part, float *dW,
-                 hipStream_t s) {
+                 float *bpart, float *db, hipStream_t s) {
+    if (bpart && !db) return -1;
     int nslots = wgrad_slots(cin, cout, g);
     int npair = (cout / 16) * ((cin + 15) / 16);
     WGRAD_CASE(8, 16)
@@ -493,17 +538,6 @@ int launch_wgrad(int cin, int cout, const NetGrid &g, const bf16_t *dy, const bf
     WGRAD_CASE(64, 32)
     WGRAD_CASE(32, 16)
     return -1;
-}
-
-int bias_slots(const NetGrid &g) {
-    long n = (g.nvox() + 255) / 256;
-    return (int)(n < 512 ? (n > 0 ? n : 1) : 512);
-}
-
-void launch_bias_grad(int cout, const NetGrid &g, const bf16_t *dy, float *part, float *db, hipStream_t s) {
-    int nslots = bias_slots(g);
-    hipLaunchKernelGGL(k_bias_part, dim3(nslots), dim3(256), 0, s, g, cout, dy, nslots, part);
-    hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(256), 0, s, part, cout, nslots, db);
 }
 
 void launch_restrict_bf16(const NetGrid &fine, const NetGrid &coarse, int C, const bf16_t *in, bf16_t *out,
