@@ -503,9 +503,11 @@ int launch_conv(int cin, int cout, const NetGrid &g, const bf16_t *in, const bf1
     return -1;
 }
 
+// workgroups per channel pair: 512 workgroups in all fill the 256 CUs at the two workgroups that
+// the weight-gradient kernel's LDS allows per CU
 int wgrad_slots(int cin, int cout, const NetGrid &g) {
     int npair = (cout / 16) * ((cin + 15) / 16);
-    long want = 1024 / npair;
+    long want = 512 / npair;
     if (want < 8) want = 8;
     return (int)(want < g.ntiles() ? want : g.ntiles());
 }
