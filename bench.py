@@ -17,6 +17,8 @@ V(2,2) red-black Gauss-Seidel cycles with vcDepth 7, stop on ||f - L u||_2 / ||f
               mixed solutions' errors against the FP64 solve, seed 0 against the oracle
   unet129     the learned guess's network on 129^3: each layer's MFMA convolution, adjoint and weight
               gradient timed, TFLOP/s against the dense BF16 peak
+  fields257   the field operators on caller arrays (include/saras_field.h) on 257^3: GB/s of each, the
+              oracle's CPU time for the same call beside it
 
 Inputs are resident in HBM before the timed region.  Prints one JSON line.  ``--gpus`` above 1 is
 refused: this build runs on one GPU.
@@ -573,6 +575,156 @@ def unet_bench(args):
     print(json.dumps(res))
 
 
+class HipTimer:
+    """HIP events on a given stream, through the HIP runtime the library itself is linked against
+    (libamdhip64.so.7 of the image), so that stream and events belong to one runtime."""
+
+    def __init__(self, stream):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.s = C.c_void_p(stream)
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        for e in (self.e0, self.e1):
+            self._ok(self.hip.hipEventCreate(C.byref(e)))
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"HIP runtime call failed with status {rc}")
+
+    def time(self, fn, reps):
+        """Average ms per call of fn (which enqueues on the stream) over reps calls."""
+        C = self.C
+        self._ok(self.hip.hipEventRecord(self.e0, self.s))
+        for _ in range(reps):
+            fn()
+        self._ok(self.hip.hipEventRecord(self.e1, self.s))
+        self._ok(self.hip.hipEventSynchronize(self.e1))
+        ms = C.c_float()
+        self._ok(self.hip.hipEventElapsedTime(C.byref(ms), self.e0, self.e1))
+        return ms.value / reps
+
+    def close(self):
+        for e in (self.e0, self.e1):
+            if e.value:
+                self.hip.hipEventDestroy(e)
+                e.value = None
+
+
+def fields_bench(args):
+    """The field operators on caller device arrays (include/saras_field.h) on the 257^3 convection
+    grid (PPN, tanh z beta 1): each operator of one time step timed with HIP events on the handle's
+    stream over --steps launches after --warmup, against its algorithmic bytes; then the Jacobi
+    viscous solve (a synchronous call: wall time and sweeps).  The CPU leg times the oracle's
+    restatement of the same operators (lib/field, one thread, -Ofast) on the same inputs."""
+    import numpy as np
+    from saras_amd import field
+    g = dict(ind=(8, 8, 8), L=(1.0, 1.0, 1.0), domain="PPN", mesh="UUD", beta=(1.0, 1.0, 1.0))
+    fg = field.FieldGrid(field.fgrid_desc(**g))
+    rng = np.random.default_rng(0)
+    shp = {loc: fg.shape(loc) for loc in range(4)}
+    host = {loc: rng.standard_normal(shp[loc]) for loc in range(4)}
+    P = fg.array(field.NODE, host[0])
+    V = [fg.array(c + 1, host[c + 1]) for c in range(3)]
+    H = [fg.array(c + 1) for c in range(3)]
+    G = [fg.array(c + 1) for c in range(3)]
+    out = fg.array(field.NODE)
+    fsz = lambda loc: float(np.prod(shp[loc]))                      # noqa: E731
+    core = lambda loc: float(np.prod([n - 2 for n in shp[loc]]))    # noqa: E731
+    w = 8.0
+    ops = [
+        ("sr_deriv1 (P, z)", lambda: fg.calcDerivative1(field.NODE, 2, P, out), 2 * w * fsz(0),
+         "F read, out written over the padded array"),
+        ("sr_deriv2 (P, z)", lambda: fg.calcDerivative2(field.NODE, 2, P, out), 2 * w * fsz(0),
+         "F read, out written over the padded array"),
+        ("sr_vf_diff", lambda: fg.vfComputeDiff(V, H), sum(3 * w * core(c + 1) for c in range(3)),
+         "per component: V_c and H_c read, H_c written"),
+        ("sr_vf_nlin (V advects V)", lambda: fg.vfComputeNLin(V, V, H), sum(5 * w * core(c + 1) for c in range(3)),
+         "per component: Vx, Vy, Vz and H_c read, H_c written"),
+        ("sr_divergence", lambda: fg.divergence(V, out), 4 * w * fsz(0),
+         "three face arrays read, the node array written"),
+        ("sr_gradient", lambda: fg.gradient(P, G), sum(2 * w * core(c + 1) for c in range(3)),
+         "per component: P read, G_c written"),
+    ]
+    timer = HipTimer(fg.stream())
+    reps = max(args.steps, 5)
+    rows = {}
+    for name, fn, nbytes, what in ops:
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        fg.sync()
+        ms = timer.time(fn, reps)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        rows[name] = {"ms": round(ms, 5), "bytes": nbytes, "bytes_are": what, "GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    timer.close()
+    # the viscous solve of Vx with the convection run's coefficients (scalar_d3 form, nu = sqrt(Pr/Ra)
+    # for Ra 1e7, Pr 1; dt 0.01): a call returns with the solve finished
+    walls = field.bc([field.PERIODIC, field.PERIODIC, field.PERIODIC, field.PERIODIC, field.DIRICHLET,
+                      field.DIRICHLET])
+    R = fg.array(field.VX, host[1])
+    Fj, Gj = fg.array(field.VX, host[1]), fg.array(field.VX)
+    t0 = time.perf_counter()
+    sweeps = fg.jacobi(field.VX, 1, 0.01, walls, R, Fj, Gj, coef=float(np.sqrt(1.0 / 1e7)), cn_tol=1e-6)
+    jac_ms = (time.perf_counter() - t0) * 1e3
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        o = po.OracleTS(po.ts_params(**g), fast=True)
+        Vh = [host[c + 1] for c in range(3)]
+        cpu_ms = {}
+        t0 = time.perf_counter()
+        o.fd_deriv(0, 1, 2, host[0])
+        cpu_ms["sr_deriv1 (P, z)"] = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        o.fd_deriv(0, 2, 2, host[0])
+        cpu_ms["sr_deriv2 (P, z)"] = (time.perf_counter() - t0) * 1e3
+        Hh = [np.zeros(shp[c + 1]) for c in range(3)]
+        t0 = time.perf_counter()
+        for c in range(3):
+            o.fd_diff(c + 1, Vh[c], Hh[c])
+        cpu_ms["sr_vf_diff"] = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        for c in range(3):
+            o.fd_nlin(c + 1, Vh[c], Vh, Hh[c])
+        cpu_ms["sr_vf_nlin (V advects V)"] = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        o.fd_divergence(Vh)
+        cpu_ms["sr_divergence"] = (time.perf_counter() - t0) * 1e3
+        Gh = [np.zeros(shp[c + 1]) for c in range(3)]
+        t0 = time.perf_counter()
+        o.fd_gradient(host[0], Gh)
+        cpu_ms["sr_gradient"] = (time.perf_counter() - t0) * 1e3
+        o.close()
+        for k, v in cpu_ms.items():
+            rows[k]["cpu_ms"] = round(v, 2)
+        cpu = {"value": round(sum(cpu_ms.values()), 2), "unit": "ms/pass", "cores": 1, "kind": "port",
+               "host": host_info(),
+               "sample": "one call of each of the six operators on the same 257^3 inputs by the oracle's "
+                         "restatement of lib/field (-Ofast, 1 thread)"}
+    P.free(), out.free()
+    for a in V + H + G + [R, Fj, Gj]:
+        a.free()
+    fg.close()
+    total = sum(r["ms"] for r in rows.values())
+    dom_name, dom = max(rows.items(), key=lambda kv: kv[1]["ms"])
+    res = {
+        "metric": "ms per pass over the field operators on 257^3 caller arrays (include/saras_field.h)",
+        "value": round(total, 4), "unit": "ms/pass", "n_gpus": 1, "steps": reps, "warmup": max(args.warmup, 1),
+        "ms_per_step": round(total, 4), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "field operators of one time step on 257^3 nodes (PPN, tanh z beta=1), seeded "
+                               "random fields on every location, HIP events on the handle's stream",
+                   "operators": rows,
+                   "jacobi_visc_Vx": {"call_ms": round(jac_ms, 3), "sweeps": int(sweeps),
+                                      "note": "synchronous: the residual is read on the host after every sweep"}},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["frac"], "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -588,7 +740,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="tgv129: time steps of the oracle's red-black step compared with the device's energy trace")
     ap.add_argument("--workload", default="rbc257",
-                    choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129", "mixed257x16", "unet129"])
+                    choices=["rbc257", "rbc257x129", "seq257", "poisson257", "tgv129", "mixed257x16", "unet129",
+                             "fields257"])
     ap.add_argument("--dt", type=float, default=0.01, help="time step of the convection workloads")
     ap.add_argument("--pad-mode", type=int, default=1, choices=[0, 1],
                     help="convection workloads: residual pads of the mixed solves' restriction (0 the reference's "
@@ -613,6 +766,9 @@ def main():
         return
     if args.workload == "unet129":
         unet_bench(args)
+        return
+    if args.workload == "fields257":
+        fields_bench(args)
         return
     if args.workload.startswith("rbc"):
         rbc_bench(args)

@@ -18,6 +18,7 @@
 #include "../../include/saras_field.h"
 #include "../../include/saras_mg.h"
 #include "ghost_waves.h"
+#include "xcd.h"
 #include "mg_state.h"
 #include "ts_kernels.h"
 
@@ -47,6 +48,7 @@ struct sr_fgrid {
     hipStream_t s = nullptr;
     double *part = nullptr;              // block maxima of the Jacobi residual
     unsigned long long *dmax = nullptr, *hmax = nullptr;
+    hipEvent_t jev = nullptr;            // marks a Jacobi residual's read-back
 };
 
 namespace {
@@ -58,28 +60,33 @@ __device__ inline long fix(const FShape &f, int i, int j, int k) {
 }
 __device__ inline double mt(const Metrics &M, int d, int node, int q, int i) { return M.m[d][node ? 0 : 1][q][i + 1]; }
 
-// padded entry t -> logical (i, j, k), each in -1..n
-__device__ inline void unflat(const FShape &f, long t, int &i, int &j, int &k) {
-    long n2 = f.n[2] + 2, n1 = f.n[1] + 2;
-    k = (int)(t % n2) - 1;
-    long r = t / n2;
-    j = (int)(r % n1) - 1;
-    i = (int)(r / n1) - 1;
+// padded entry t -> logical (i, j, k), each in -1..n (32-bit: a padded array has < 2^32 entries)
+__device__ inline void unflat(const FShape &f, unsigned t, int &i, int &j, int &k) {
+    unsigned n2 = (unsigned)f.n[2] + 2u, n1 = (unsigned)f.n[1] + 2u;
+    unsigned r = t / n2;
+    k = (int)(t - r * n2) - 1;
+    unsigned q = r / n1;
+    j = (int)(r - q * n1) - 1;
+    i = (int)q - 1;
 }
 // core entry t -> logical (i, j, k), each in 0..n-1
-__device__ inline void unflat_core(const FShape &f, long t, int &i, int &j, int &k) {
-    k = (int)(t % f.n[2]);
-    long r = t / f.n[2];
-    j = (int)(r % f.n[1]);
-    i = (int)(r / f.n[1]);
+__device__ inline void unflat_core(const FShape &f, unsigned t, int &i, int &j, int &k) {
+    unsigned n2 = (unsigned)f.n[2], n1 = (unsigned)f.n[1];
+    unsigned r = t / n2;
+    k = (int)(t - r * n2);
+    unsigned q = r / n1;
+    j = (int)(r - q * n1);
+    i = (int)q;
 }
+// the thread's entry: logical workgroups in XCD order (each XCD a contiguous range of rows)
+__device__ inline unsigned entry() { return xcd_block() * 256u + threadIdx.x; }
 
 // calcDerivative1 / 2 along d over the whole padded array: the core range along d gets the
 // derivative, the two pad layers along d get 0
 __global__ void __launch_bounds__(256) k_fd_deriv(FShape f, Metrics M, const double *__restrict__ F,
-                                                  double *__restrict__ out, int order, int d, double h, double half) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fsz(f)) return;
+                                                  double *__restrict__ out, int order, int d, double ih, double half) {
+    const unsigned t = entry();
+    if ((long)t >= fsz(f)) return;
     int idx[3];
     unflat(f, t, idx[0], idx[1], idx[2]);
     int r = idx[d];
@@ -88,7 +95,6 @@ __global__ void __launch_bounds__(256) k_fd_deriv(FShape f, Metrics M, const dou
         return;
     }
     long st = d == 0 ? (long)(f.n[1] + 2) * (f.n[2] + 2) : d == 1 ? (long)(f.n[2] + 2) : 1;
-    double ih = 1.0 / h;
     double fp = F[t + st], fm = F[t - st];
     if (order == 1) {
         double v = (fp - fm) / 2.0;
@@ -104,20 +110,20 @@ __global__ void __launch_bounds__(256) k_fd_deriv(FShape f, Metrics M, const dou
 
 // computeDiff of one field: H(core) += d2xx, += d2yy, += d2zz in that order
 __global__ void __launch_bounds__(256) k_fd_diff(FShape f, Metrics M, const double *__restrict__ F,
-                                                 double *__restrict__ H, double h0, double h1, double h2,
+                                                 double *__restrict__ H, double ih0, double ih1, double ih2,
                                                  double half) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)f.n[0] * f.n[1] * f.n[2]) return;
+    const unsigned t = entry();
+    if ((long)t >= (long)f.n[0] * f.n[1] * f.n[2]) return;
     int idx[3];
     unflat_core(f, t, idx[0], idx[1], idx[2]);
     long q = fix(f, idx[0], idx[1], idx[2]);
     long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
-    double hh[3] = {h0, h1, h2};
+    double ihh[3] = {ih0, ih1, ih2};
     double acc = H[q];
     double f0 = F[q];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double ih = 1.0 / hh[d];
+        double ih = ihh[d];
         double ih2 = ih * ih;
         double fp = F[q + st[d]], fm = F[q - st[d]];
         double tmp = (fp - fm) / 2.0 * ih;
@@ -135,15 +141,15 @@ __global__ void __launch_bounds__(256) k_fd_diff(FShape f, Metrics M, const doub
 __global__ void __launch_bounds__(256) k_fd_nlin(int loc, FShape f, FShape fx, FShape fy, FShape fz, Metrics M,
                                                  const double *__restrict__ F, const double *__restrict__ Vx,
                                                  const double *__restrict__ Vy, const double *__restrict__ Vz,
-                                                 double *__restrict__ H, double h0, double h1, double h2) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)f.n[0] * f.n[1] * f.n[2]) return;
+                                                 double *__restrict__ H, double ih0, double ih1, double ih2) {
+    const unsigned t = entry();
+    if ((long)t >= (long)f.n[0] * f.n[1] * f.n[2]) return;
     int i, j, k;
     unflat_core(f, t, i, j, k);
     int idx[3] = {i, j, k};
     long q = fix(f, i, j, k);
     long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
-    double hh[3] = {h0, h1, h2};
+    double ihh[3] = {ih0, ih1, ih2};
     const double *V[3] = {Vx, Vy, Vz};
     const FShape *FS[3] = {&fx, &fy, &fz};
     const int c = loc - 1;
@@ -175,11 +181,12 @@ __global__ void __launch_bounds__(256) k_fd_nlin(int loc, FShape f, FShape fx, F
             inter += Vd[fix(fd, o3[0], o3[1], o3[2])];
             ns = 4;
         }
-        double ih = 1.0 / hh[d];
+        double ih = ihh[d];
         double dv = (F[q + st[d]] - F[q - st[d]]) / 2.0;
         dv *= ih;
         double deriv = mt(M, d, f.s[d], 1, idx[d]) * dv;
-        acc -= inter * deriv / (double)ns;
+        // / ns with ns = 1, 2 or 4: the same as * (1 / ns), exactly
+        acc -= inter * deriv * (ns == 1 ? 1.0 : ns == 2 ? 0.5 : 0.25);
     }
     H[q] = acc;
 }
@@ -187,8 +194,8 @@ __global__ void __launch_bounds__(256) k_fd_nlin(int loc, FShape f, FShape fx, F
 // gradient component c on its core: G = xi_c(face) (F(+1) - F) / h_c
 __global__ void __launch_bounds__(256) k_fd_gradient(int c, FShape fc, FShape fp, Metrics M,
                                                      const double *__restrict__ F, double *__restrict__ G, double hc) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)fc.n[0] * fc.n[1] * fc.n[2]) return;
+    const unsigned t = entry();
+    if ((long)t >= (long)fc.n[0] * fc.n[1] * fc.n[2]) return;
     int i, j, k;
     unflat_core(fc, t, i, j, k);
     int idx[3] = {i, j, k};
@@ -202,8 +209,8 @@ __global__ void __launch_bounds__(256) k_fd_divergence(FShape fp, FShape fx, FSh
                                                        const double *__restrict__ Vx, const double *__restrict__ Vy,
                                                        const double *__restrict__ Vz, double *__restrict__ out,
                                                        double h0, double h1, double h2) {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fsz(fp)) return;
+    const unsigned t = entry();
+    if ((long)t >= fsz(fp)) return;
     int i, j, k;
     unflat(fp, t, i, j, k);
     if (i < 0 || j < 0 || k < 0 || i >= fp.n[0] || j >= fp.n[1] || k >= fp.n[2]) {
@@ -311,7 +318,10 @@ static int fgrid_create_impl(const sr_fgrid_desc *desc, sr_fgrid *g) {
                 g->M.m[k][face][q] = g->dmet + off;
                 off += v.size();
             }
-    HIPCHK(hipMalloc((void **)&g->part, sizeof(double) * ts_jacobi_res_parts()));
+    long nparts = ts_jacobi_res_parts();
+    for (int loc = 0; loc < 4; loc++) nparts = std::max(nparts, ts_jacobi_vres_parts(g->shape[loc]));
+    HIPCHK(hipMalloc((void **)&g->part, sizeof(double) * nparts));
+    HIPCHK(hipEventCreateWithFlags(&g->jev, hipEventDisableTiming));
     HIPCHK(hipMalloc((void **)&g->dmax, sizeof(unsigned long long)));
     HIPCHK(hipHostMalloc((void **)&g->hmax, sizeof(unsigned long long), hipHostMallocDefault));
     return SR_OK;
@@ -335,6 +345,7 @@ extern "C" int sr_fgrid_destroy(sr_fgrid *g) {
     if (!g) return SR_OK;
     if (g->s) (void)hipStreamSynchronize(g->s);
     if (g->dmet) (void)hipFree(g->dmet);
+    if (g->jev) (void)hipEventDestroy(g->jev);
     if (g->part) (void)hipFree(g->part);
     if (g->dmax) (void)hipFree(g->dmax);
     if (g->hmax) (void)hipHostFree(g->hmax);
@@ -374,8 +385,9 @@ static int deriv(sr_fgrid *g, int loc, int order, int dim, const double *F, doub
     if (!F || !out) return set_err(SR_ERR_ARG, "null array");
     if ((const double *)out == F) return set_err(SR_ERR_ARG, "out must not alias F");
     const FShape &f = g->shape[loc];
-    hipLaunchKernelGGL(k_fd_deriv, dim3(nblocks(fsz(f))), dim3(256), 0, g->s, f, g->M, F, out, order, dim, g->h[dim],
-                       g->half);
+    // the inverse grid spacings are formed on the host, like derivative's ihx, ihy, ihz (derivative.cc:66-68)
+    hipLaunchKernelGGL(k_fd_deriv, dim3(nblocks(fsz(f))), dim3(256), 0, g->s, f, g->M, F, out, order, dim,
+                       1.0 / g->h[dim], g->half);
     HIPCHK(hipGetLastError());
     return SR_OK;
 }
@@ -392,8 +404,8 @@ static int diff(sr_fgrid *g, int loc, const double *F, double *H) {
     if ((const double *)H == F) return set_err(SR_ERR_ARG, "H must not alias the field");
     const FShape &f = g->shape[loc];
     long n = (long)f.n[0] * f.n[1] * f.n[2];
-    hipLaunchKernelGGL(k_fd_diff, dim3(nblocks(n)), dim3(256), 0, g->s, f, g->M, F, H, g->h[0], g->h[1], g->h[2],
-                       g->half);
+    hipLaunchKernelGGL(k_fd_diff, dim3(nblocks(n)), dim3(256), 0, g->s, f, g->M, F, H, 1.0 / g->h[0], 1.0 / g->h[1],
+                       1.0 / g->h[2], g->half);
     HIPCHK(hipGetLastError());
     return SR_OK;
 }
@@ -419,8 +431,8 @@ static int nlin(sr_fgrid *g, int loc, const double *F, const double *const V[3],
     const FShape &f = g->shape[loc];
     long n = (long)f.n[0] * f.n[1] * f.n[2];
     hipLaunchKernelGGL(k_fd_nlin, dim3(nblocks(n)), dim3(256), 0, g->s, loc, f, g->shape[SR_LOC_VX],
-                       g->shape[SR_LOC_VY], g->shape[SR_LOC_VZ], g->M, F, V[0], V[1], V[2], H, g->h[0], g->h[1],
-                       g->h[2]);
+                       g->shape[SR_LOC_VY], g->shape[SR_LOC_VZ], g->M, F, V[0], V[1], V[2], H, 1.0 / g->h[0],
+                       1.0 / g->h[1], 1.0 / g->h[2]);
     HIPCHK(hipGetLastError());
     return SR_OK;
 }
@@ -525,14 +537,19 @@ extern "C" int sr_jacobi_visc(sr_fgrid *g, int loc, const sr_jacobi_desc *jd, co
     int sweeps = 0;
     double *a = F, *b = G;
     int status = SR_OK;
+    // the first sweep, then one launch per iteration: the next iterate into b and the residual of
+    // the current one in a from the same loads; b gets its BCs while the host reads the residual,
+    // and is dropped once a has converged (iterates, residuals and sweep counts unchanged)
+    ts_jacobi(f, g->M, B, a, R, b, J, g->s);
+    std::swap(a, b);
+    for (const GhostWave &w : ws) ts_ghost_wave(f, a, w, g->s);
     while (true) {
-        ts_jacobi(f, g->M, B, a, R, b, J, g->s);
-        std::swap(a, b);
-        for (const GhostWave &w : ws) ts_ghost_wave(f, a, w, g->s);
-        ts_jacobi_res(f, g->M, B, a, R, J, g->part, g->dmax, g->s);
+        ts_jacobi_vres(f, g->M, B, a, R, b, J, g->part, g->dmax, g->s);
         HIPCHK(hipMemcpyAsync(g->hmax, g->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
-        HIPCHK(hipStreamSynchronize(g->s));
+        HIPCHK(hipEventRecord(g->jev, g->s));
+        for (const GhostWave &w : ws) ts_ghost_wave(f, b, w, g->s);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(g->jev));
         double maxError;
         unsigned long long bits = *g->hmax;
         std::memcpy(&maxError, &bits, sizeof(double));
@@ -543,12 +560,12 @@ extern "C" int sr_jacobi_visc(sr_fgrid *g, int loc, const sr_jacobi_desc *jd, co
             status = set_err(SR_ERR_NOT_CONVERGED, "Jacobi iterations did not converge within the iteration limit");
             break;
         }
+        std::swap(a, b);
     }
-    // the result lives in whichever buffer the last sweep wrote
-    if (a != F) {
-        HIPCHK(hipMemcpyAsync(F, a, sizeof(double) * fsz(f), hipMemcpyDeviceToDevice, g->s));
-        HIPCHK(hipStreamSynchronize(g->s));
-    }
+    // the result lives in whichever buffer holds the converged iterate; the dropped iterate's BCs
+    // are still on the stream, so the call waits for them too before G goes back to the caller
+    if (a != F) HIPCHK(hipMemcpyAsync(F, a, sizeof(double) * fsz(f), hipMemcpyDeviceToDevice, g->s));
+    HIPCHK(hipStreamSynchronize(g->s));
     if (iters) *iters = sweeps;
     return status;
 }

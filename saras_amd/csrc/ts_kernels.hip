@@ -34,7 +34,7 @@ __device__ inline double metv(const Metrics &M, int d, int node, int q, int i) {
 // derivatives (for parity tests; the fused step kernel carries its own copies)
 // ------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_deriv(FShape f, Metrics M, const double *__restrict__ F,
-                                               double *__restrict__ out, int order, int d, double h) {
+                                               double *__restrict__ out, int order, int d, double ih) {
     int lo[3] = {-1, -1, -1}, sz[3] = {f.n[0] + 2, f.n[1] + 2, f.n[2] + 2};
     lo[d] = 0;
     sz[d] = f.n[d];
@@ -48,7 +48,6 @@ __global__ void __launch_bounds__(256) k_deriv(FShape f, Metrics M, const double
     long st[3] = {(long)(f.n[1] + 2) * (f.n[2] + 2), (long)(f.n[2] + 2), 1};
     long c = fidx(f, i, j, k);
     int q = d == 0 ? i : d == 1 ? j : k;
-    double ih = 1.0 / h;
     if (order == 1) {
         double v = (F[c + st[d]] - F[c - st[d]]) / 2.0;
         v *= ih;
@@ -68,7 +67,8 @@ __global__ void __launch_bounds__(256) k_build_rhs(int c, FShape fc, FShape fx, 
                                                    Metrics M, const double *__restrict__ Vx,
                                                    const double *__restrict__ Vy, const double *__restrict__ Vz,
                                                    const double *__restrict__ P, double *__restrict__ H, double h0,
-                                                   double h1, double h2, double inverseRe, double dt,
+                                                   double h1, double h2, double ih0, double ih1, double ih2,
+                                                   double inverseRe, double dt,
                                                    const double *__restrict__ T, double Fb) {
     // one thread per padded entry: core entries get the right-hand side, pads a copy of V (the
     // reference forms nseRHS over the whole padded array, with zero terms outside the core)
@@ -90,12 +90,12 @@ __global__ void __launch_bounds__(256) k_build_rhs(int c, FShape fc, FShape fx, 
     long st[3] = {(long)(fc.n[1] + 2) * (fc.n[2] + 2), (long)(fc.n[2] + 2), 1};
     long q0 = t;
     int idx[3] = {i, j, k};
-    double hh[3] = {h0, h1, h2};
+    double hh[3] = {h0, h1, h2}, ihh[3] = {ih0, ih1, ih2};
     // computeDiff: H = d2xx + d2yy + d2zz (each added to the zero-initialised nseRHS), then / Re
     double acc = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double ih = 1.0 / hh[d];
+        double ih = ihh[d];
         double ih2 = ih * ih;
         double fp_ = F[q0 + st[d]], fm_ = F[q0 - st[d]], f0 = F[q0];
         double tmp = (fp_ - fm_) / 2.0 * ih;
@@ -126,11 +126,12 @@ __global__ void __launch_bounds__(256) k_build_rhs(int c, FShape fc, FShape fx, 
             inter += Vd[fidx(fd, o3[0], o3[1], o3[2])];
             ns = 4;
         }
-        double ih = 1.0 / hh[d];
+        double ih = ihh[d];
         double dv = (F[q0 + st[d]] - F[q0 - st[d]]) / 2.0;
         dv *= ih;
         double deriv = metv(M, d, fc.s[d], 1, idx[d]) * dv;
-        acc -= inter * deriv / (double)ns;
+        // / ns with ns = 1 or 4: the same as * (1 / ns), exactly
+        acc -= inter * deriv * (ns == 1 ? 1.0 : 0.25);
     }
     long sp[3] = {(long)(fp.n[1] + 2) * (fp.n[2] + 2), (long)(fp.n[2] + 2), 1};
     long pq = fidx(fp, i, j, k);
@@ -263,17 +264,22 @@ __global__ void __launch_bounds__(256) k_jacobi_vres(FShape f, Metrics M, Bulk B
     if (threadIdx.x == 0) part[lb] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
 }
 
-// maximum of the block maxima, stored as the bit pattern of the double
-__global__ void __launch_bounds__(256) k_jacobi_res_final(const double *__restrict__ part, int n,
-                                                          unsigned long long *__restrict__ out) {
+// maximum of the block maxima, stored as the bit pattern of the double; one block of up to 1024
+// threads (a maximum: the same whatever the order)
+__global__ void __launch_bounds__(1024) k_jacobi_res_final(const double *__restrict__ part, int n,
+                                                           unsigned long long *__restrict__ out) {
     double m = 0.0;
     for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
-    __shared__ double s[4];
+    __shared__ double s[16];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = (unsigned long long)__double_as_longlong(fmax(fmax(s[0], s[1]), fmax(s[2], s[3])));
+    if (threadIdx.x == 0) {
+        double r = s[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = fmax(r, s[w]);
+        out[0] = (unsigned long long)__double_as_longlong(r);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -312,7 +318,7 @@ __global__ void __launch_bounds__(256) k_ghost_ops(FShape f, double *__restrict_
 __global__ void __launch_bounds__(256) k_build_trhs(FShape fp, FShape fx, FShape fy, FShape fz, Metrics M,
                                                     const double *__restrict__ Vx, const double *__restrict__ Vy,
                                                     const double *__restrict__ Vz, const double *__restrict__ T,
-                                                    double *__restrict__ R, double h0, double h1, double h2,
+                                                    double *__restrict__ R, double ih0, double ih1, double ih2,
                                                     double kappa, double dt) {
     long total = (long)(fp.n[0] + 2) * (fp.n[1] + 2) * (fp.n[2] + 2);
     const unsigned lb = xcd_block();
@@ -328,13 +334,13 @@ __global__ void __launch_bounds__(256) k_build_trhs(FShape fp, FShape fx, FShape
     }
     long st[3] = {(long)(fp.n[1] + 2) * (fp.n[2] + 2), (long)(fp.n[2] + 2), 1};
     int idx[3] = {i, j, k};
-    double hh[3] = {h0, h1, h2};
+    double ihh[3] = {ih0, ih1, ih2};
     const double *V[3] = {Vx, Vy, Vz};
     const FShape *FS[3] = {&fx, &fy, &fz};
     double acc = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double ih = 1.0 / hh[d];
+        double ih = ihh[d];
         double ih2 = ih * ih;
         double fpv = T[t + st[d]], fmv = T[t - st[d]], f0 = T[t];
         double tmp = (fpv - fmv) / 2.0 * ih;
@@ -349,7 +355,7 @@ __global__ void __launch_bounds__(256) k_build_trhs(FShape fp, FShape fx, FShape
         double inter = 0.0;
         inter += V[d][fidx(*FS[d], i, j, k)];
         inter += V[d][fidx(*FS[d], o[0], o[1], o[2])];
-        double ih = 1.0 / hh[d];
+        double ih = ihh[d];
         double dv = (T[t + st[d]] - T[t - st[d]]) / 2.0;
         dv *= ih;
         double deriv = metv(M, d, 1, 1, idx[d]) * dv;
@@ -566,7 +572,7 @@ void ts_deriv(const FShape &f, const Metrics &M, const double *F, double *out, i
               hipStream_t s) {
     long sz[3] = {f.n[0] + 2L, f.n[1] + 2L, f.n[2] + 2L};
     sz[d] = f.n[d];
-    hipLaunchKernelGGL(k_deriv, dim3(nb(sz[0] * sz[1] * sz[2])), dim3(256), 0, s, f, M, F, out, order, d, h);
+    hipLaunchKernelGGL(k_deriv, dim3(nb(sz[0] * sz[1] * sz[2])), dim3(256), 0, s, f, M, F, out, order, d, 1.0 / h);
 }
 
 void ts_build_rhs(int c, const FShape *fv, const FShape &fp, const Metrics &M, const double *const *V,
@@ -574,15 +580,16 @@ void ts_build_rhs(int c, const FShape *fv, const FShape &fp, const Metrics &M, c
                   hipStream_t s) {
     const FShape &fc = fv[c];
     long n = (long)(fc.n[0] + 2) * (fc.n[1] + 2) * (fc.n[2] + 2);
+    // the inverse grid spacings are formed on the host, like derivative's ihx, ihy, ihz (derivative.cc:66-68)
     hipLaunchKernelGGL(k_build_rhs, dim3(nb(n)), dim3(256), 0, s, c, fc, fv[0], fv[1], fv[2], fp, M, V[0], V[1],
-                       V[2], P, H, h[0], h[1], h[2], visc, dt, T, Fb);
+                       V[2], P, H, h[0], h[1], h[2], 1.0 / h[0], 1.0 / h[1], 1.0 / h[2], visc, dt, T, Fb);
 }
 
 void ts_build_trhs(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, const double *T,
                    double *R, const double *h, double kappa, double dt, hipStream_t s) {
     long n = (long)(fp.n[0] + 2) * (fp.n[1] + 2) * (fp.n[2] + 2);
     hipLaunchKernelGGL(k_build_trhs, dim3(nb(n)), dim3(256), 0, s, fp, fv[0], fv[1], fv[2], M, V[0], V[1], V[2], T, R,
-                       h[0], h[1], h[2], kappa, dt);
+                       1.0 / h[0], 1.0 / h[1], 1.0 / h[2], kappa, dt);
 }
 
 void ts_nusselt(const FShape &fp, const FShape &fz, const Metrics &M, const double *Vz, const double *T,
@@ -624,7 +631,7 @@ void ts_jacobi_res(const FShape &f, const Metrics &M, const Bulk &B, const doubl
     if (g > (unsigned)ts_jacobi_res_parts()) g = (unsigned)ts_jacobi_res_parts();
     if (g == 0) g = 1;
     hipLaunchKernelGGL(k_jacobi_res, dim3(g), dim3(256), 0, s, f, M, B, V, R, with_quotients(J), part);
-    hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(256), 0, s, part, (int)g, out);
+    hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(1024), 0, s, part, (int)g, out);
 }
 
 long ts_jacobi_vres_parts(const FShape &f) {
@@ -635,7 +642,7 @@ void ts_jacobi_vres(const FShape &f, const Metrics &M, const Bulk &B, const doub
                     const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s) {
     unsigned g = (unsigned)ts_jacobi_vres_parts(f);
     hipLaunchKernelGGL(k_jacobi_vres, dim3(g), dim3(256), 0, s, f, M, B, V, R, G, with_quotients(J), part);
-    hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(256), 0, s, part, (int)g, out);
+    hipLaunchKernelGGL(k_jacobi_res_final, dim3(1), dim3(1024), 0, s, part, (int)g, out);
 }
 
 void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s) {

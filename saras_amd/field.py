@@ -67,6 +67,12 @@ class FieldGrid:
     def sync(self):
         _L.check(self.L.sr_fgrid_sync(self.h))
 
+    def stream(self):
+        """The handle's HIP stream (hipStream_t as an integer), on which the operators run."""
+        st = C.c_void_p()
+        _L.check(self.L.sr_fgrid_stream(self.h, C.byref(st)))
+        return st.value
+
     # derivative (derivative.h:78-84)
     def calcDerivative1(self, loc, dim, F, out):
         _L.check(self.L.sr_deriv1(self.h, loc, dim, F.ptr, out.ptr))
