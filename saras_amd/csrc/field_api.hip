@@ -509,59 +509,23 @@ extern "C" int sr_jacobi_visc(sr_fgrid *g, int loc, const sr_jacobi_desc *jd, co
     const FShape &f = g->shape[loc];
     std::vector<GhostWave> ws;
     RETCHK(bc_waves(g, f, bc, ws));
-    JacobiCoef J{};
-    J.hx = g->h[0]; J.hy = g->h[1]; J.hz = g->h[2];
-    J.hx2hy2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0);
-    J.hy2hz2 = std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
-    J.hz2hx2 = std::pow(J.hz, 2.0) * std::pow(J.hx, 2.0);
-    J.hx2hy2hz2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
-    J.dt = jd->dt;
-    J.Re = jd->Re;
-    J.inverseRe = jd->form == 0 ? 1.0 / jd->Re : 0.0;
+    JacobiCoef J = jacobi_coef(g->h, jd->dt, jd->form == 0 ? jd->Re : 0.0);
     J.vis = jd->coef;
     J.form = jd->form;
     for (int k = 0; k < 3; k++) J.ms[k] = f.s[k];
     if (jd->zCollocMetric) J.ms[2] = 0;
     int per[3] = {g->d.xPer, g->d.yPer, g->d.zPer};
-    Bulk B{};
-    for (int k = 0; k < 3; k++) {
-        B.lo[k] = 0;
-        B.hi[k] = f.n[k] - 1;
-        if (f.s[k] && !per[k]) {
-            B.lo[k] = 1;
-            B.hi[k] = f.n[k] - 2;
-        }
-    }
+    Bulk B = jacobi_bulk(f, per);
     long maxIterations = (long)(g->N[0] - 1) * (g->N[1] - 1) * (g->N[2] - 1);
-    long iterCount = 0;
-    int sweeps = 0;
     double *a = F, *b = G;
+    JacobiIO io{g->part, g->dmax, g->hmax, g->jev, g->s};
+    int sweeps = 0;
+    hipError_t err = hipSuccess;
+    int st = ts_jacobi_loop(f, g->M, B, a, b, R, J, ws.data(), (int)ws.size(), io, jd->cnTolerance, maxIterations,
+                            &sweeps, &err);
+    HIPCHK(err);
     int status = SR_OK;
-    // the first sweep, then one launch per iteration: the next iterate into b and the residual of
-    // the current one in a from the same loads; b gets its BCs while the host reads the residual,
-    // and is dropped once a has converged (iterates, residuals and sweep counts unchanged)
-    ts_jacobi(f, g->M, B, a, R, b, J, g->s);
-    std::swap(a, b);
-    for (const GhostWave &w : ws) ts_ghost_wave(f, a, w, g->s);
-    while (true) {
-        ts_jacobi_vres(f, g->M, B, a, R, b, J, g->part, g->dmax, g->s);
-        HIPCHK(hipMemcpyAsync(g->hmax, g->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->s));
-        HIPCHK(hipEventRecord(g->jev, g->s));
-        for (const GhostWave &w : ws) ts_ghost_wave(f, b, w, g->s);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventSynchronize(g->jev));
-        double maxError;
-        unsigned long long bits = *g->hmax;
-        std::memcpy(&maxError, &bits, sizeof(double));
-        sweeps += 1;
-        if (maxError < jd->cnTolerance) break;
-        iterCount += 1;
-        if (iterCount > maxIterations) {
-            status = set_err(SR_ERR_NOT_CONVERGED, "Jacobi iterations did not converge within the iteration limit");
-            break;
-        }
-        std::swap(a, b);
-    }
+    if (st == 1) status = set_err(SR_ERR_NOT_CONVERGED, "Jacobi iterations did not converge within the iteration limit");
     // the result lives in whichever buffer holds the converged iterate; the dropped iterate's BCs
     // are still on the stream, so the call waits for them too before G goes back to the caller
     if (a != F) HIPCHK(hipMemcpyAsync(F, a, sizeof(double) * fsz(f), hipMemcpyDeviceToDevice, g->s));

@@ -427,62 +427,26 @@ extern "C" int sr_ts_build_rhs(sr_ts *t) {
     return SR_OK;
 }
 
-static JacobiCoef jacobi_coef(const sr_ts *t) {
-    JacobiCoef J{};
-    J.hx = t->h[0]; J.hy = t->h[1]; J.hz = t->h[2];
-    J.hx2hy2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0);
-    J.hy2hz2 = std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
-    J.hz2hx2 = std::pow(J.hz, 2.0) * std::pow(J.hx, 2.0);
-    J.hx2hy2hz2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
-    J.dt = t->d.dt;
-    J.Re = t->d.Re;
-    J.inverseRe = t->d.Re > 0.0 ? 1.0 / t->d.Re : 0.0;
-    return J;
-}
+static JacobiCoef jacobi_coef(const sr_ts *t) { return jacobi_coef(t->h, t->d.dt, t->d.Re); }
 
 // Jacobi sweeps of (1 - dt vis lap / 2) F = R until the maximum residual over the bulk is below
-// cnTolerance; the reference aborts after (Nx-1)(Ny-1)(Nz-1) sweeps without convergence.  F and G
-// swap after every sweep (G is written everywhere: 0 outside the bulk, as the reference's guess
-// field), then the field's BCs are imposed.  After the first sweep, one launch per iteration makes
-// the next iterate in G and the residual of the current one in F (the same loads serve both); the
-// next iterate gets its BCs while the host reads the residual, and is dropped once F has converged,
-// so F, the residuals and the sweep count are those of the sweep-then-check loop.
+// cnTolerance (ts_jacobi_loop); the reference aborts after (Nx-1)(Ny-1)(Nz-1) sweeps without
+// convergence.  F and G swap as the iterates move (G is written everywhere: 0 outside the bulk, as
+// the reference's guess field).
 static int jacobi(sr_ts *t, const FShape &f, double *&F, double *&G, const double *R, const JacobiCoef &J,
                   const std::vector<GhostWave> &bc, const char *what, int *iters) {
     int per[3] = {t->d.xPer, t->d.yPer, t->d.zPer};
-    Bulk B{};
-    for (int k = 0; k < 3; k++) {
-        B.lo[k] = 0;
-        B.hi[k] = f.n[k] - 1;
-        if (f.s[k] && !per[k]) {
-            B.lo[k] = 1;
-            B.hi[k] = f.n[k] - 2;
-        }
-    }
+    Bulk B = jacobi_bulk(f, per);
     long maxIterations = (long)(t->N[0] - 1) * (t->N[1] - 1) * (t->N[2] - 1);
-    long iterCount = 0;
+    JacobiIO io{t->part, t->dmax, t->hmax, t->jev, t->s};
     int sweeps = 0;
-    ts_jacobi(f, t->M, B, F, R, G, J, t->s);
-    std::swap(F, G);
-    for (const GhostWave &w : bc) ts_ghost_wave(f, F, w, t->s);
-    while (true) {
-        ts_jacobi_vres(f, t->M, B, F, R, G, J, t->part, t->dmax, t->s);
-        HIPCHK(hipMemcpyAsync(t->hmax, t->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, t->s));
-        HIPCHK(hipEventRecord(t->jev, t->s));
-        for (const GhostWave &w : bc) ts_ghost_wave(f, G, w, t->s);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventSynchronize(t->jev));
-        double maxError;
-        unsigned long long bits = *t->hmax;
-        std::memcpy(&maxError, &bits, sizeof(double));
-        sweeps += 1;
-        if (maxError < t->d.cnTolerance) break;
-        iterCount += 1;
-        if (iterCount > maxIterations)
-            return set_err(SR_ERR_NOT_CONVERGED, std::string("Jacobi iterations for ") + what +
-                                                     " did not converge within the iteration limit");
-        std::swap(F, G);
-    }
+    hipError_t err = hipSuccess;
+    int st = ts_jacobi_loop(f, t->M, B, F, G, R, J, bc.data(), (int)bc.size(), io, t->d.cnTolerance, maxIterations,
+                            &sweeps, &err);
+    HIPCHK(err);
+    if (st == 1)
+        return set_err(SR_ERR_NOT_CONVERGED, std::string("Jacobi iterations for ") + what +
+                                                 " did not converge within the iteration limit");
     if (iters) *iters = sweeps;
     return SR_OK;
 }

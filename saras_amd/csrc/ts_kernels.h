@@ -81,6 +81,35 @@ long ts_jacobi_vres_parts(const FShape &f);
 void ts_jacobi_vres(const FShape &f, const Metrics &M, const Bulk &B, const double *V, const double *R, double *G,
                     const JacobiCoef &J, double *part, unsigned long long *out, hipStream_t s);
 void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s);
+
+// The grid factors of the Jacobi solves (hydro_d3.cc:56-61: hx2hy2 = hx^2 hy^2 ...) for spacings h,
+// time step dt and Reynolds number Re (1 / Re = 0 when Re <= 0); form, vis and ms are the caller's.
+JacobiCoef jacobi_coef(const double h[3], double dt, double Re);
+
+// The bulk of a Jacobi solve (field.cc:206-219): every core entry, except that a non-periodic
+// direction in which the field sits on nodes drops its two end nodes.
+Bulk jacobi_bulk(const FShape &f, const int per[3]);
+
+// The Jacobi loop of solveVx/Vy/Vz and solveT (hydro_d3.cc:370-533, scalar_d3.cc:373-594):
+// sweep, the BC waves, the residual maximum over the bulk, until it is below tol.  After the first
+// sweep one launch per iteration makes the next iterate in G and the residual of the current one in
+// F from the same loads (ts_jacobi_vres); the next iterate gets its BCs while the host reads the
+// residual through hmax and the event, and is dropped once F has converged, so F, the residuals and
+// the sweep count are those of the sweep-then-check loop.  F and G are swapped as the iterates
+// move; on return F holds the converged iterate, or with status 1 the last one whose residual was
+// read after max_iter + 1 checks without convergence.  Returns 0 or 1, or -1 with *err set when a
+// HIP call failed.  part holds max(ts_jacobi_vres_parts(f), 1) doubles; G's BC waves may still be
+// queued on s on return.
+struct JacobiIO {
+    double *part;
+    unsigned long long *dmax, *hmax;   // device maximum and its pinned host mirror
+    hipEvent_t ev;
+    hipStream_t s;
+};
+int ts_jacobi_loop(const FShape &f, const Metrics &M, const Bulk &B, double *&F, double *&G, const double *R,
+                   const JacobiCoef &J, const GhostWave *bc, int nbc, const JacobiIO &io, double tol, long max_iter,
+                   int *sweeps, hipError_t *err);
+
 void ts_divergence(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, double *out,
                    const double *h, double scale, int scaled, hipStream_t s);
 void ts_project(int c, const FShape &fc, const FShape &fp, const Metrics &M, const double *Pp, double *V,

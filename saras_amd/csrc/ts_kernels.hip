@@ -16,6 +16,10 @@
 // out(r) = metric(r) (F(r+1) - F(r-1)) / (2h) over the core range along the direction.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstring>
+#include <utility>
+
 #include "ts_kernels.h"
 #include "xcd.h"
 
@@ -649,6 +653,75 @@ void ts_ghost_wave(const FShape &f, double *F, const GhostWave &W, hipStream_t s
     long n = 0;
     for (int o = 0; o < W.nops; o++) n += W.ops[o].count;
     if (n > 0) hipLaunchKernelGGL(k_ghost_ops, dim3(nb(n)), dim3(256), 0, s, f, F, W);
+}
+
+JacobiCoef jacobi_coef(const double h[3], double dt, double Re) {
+    JacobiCoef J{};
+    J.hx = h[0];
+    J.hy = h[1];
+    J.hz = h[2];
+    J.hx2hy2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0);
+    J.hy2hz2 = std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
+    J.hz2hx2 = std::pow(J.hz, 2.0) * std::pow(J.hx, 2.0);
+    J.hx2hy2hz2 = std::pow(J.hx, 2.0) * std::pow(J.hy, 2.0) * std::pow(J.hz, 2.0);
+    J.dt = dt;
+    J.Re = Re;
+    J.inverseRe = Re > 0.0 ? 1.0 / Re : 0.0;
+    return J;
+}
+
+Bulk jacobi_bulk(const FShape &f, const int per[3]) {
+    Bulk B{};
+    for (int k = 0; k < 3; k++) {
+        B.lo[k] = 0;
+        B.hi[k] = f.n[k] - 1;
+        if (f.s[k] && !per[k]) {
+            B.lo[k] = 1;
+            B.hi[k] = f.n[k] - 2;
+        }
+    }
+    return B;
+}
+
+int ts_jacobi_loop(const FShape &f, const Metrics &M, const Bulk &B, double *&F, double *&G, const double *R,
+                   const JacobiCoef &J, const GhostWave *bc, int nbc, const JacobiIO &io, double tol, long max_iter,
+                   int *sweeps, hipError_t *err) {
+#define JCHK(x)                      \
+    do {                             \
+        hipError_t e_ = (x);         \
+        if (e_ != hipSuccess) {      \
+            *err = e_;               \
+            return -1;               \
+        }                            \
+    } while (0)
+    long iterCount = 0;
+    int n = 0;
+    ts_jacobi(f, M, B, F, R, G, J, io.s);
+    std::swap(F, G);
+    for (int w = 0; w < nbc; w++) ts_ghost_wave(f, F, bc[w], io.s);
+    int status = 0;
+    while (true) {
+        ts_jacobi_vres(f, M, B, F, R, G, J, io.part, io.dmax, io.s);
+        JCHK(hipMemcpyAsync(io.hmax, io.dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, io.s));
+        JCHK(hipEventRecord(io.ev, io.s));
+        for (int w = 0; w < nbc; w++) ts_ghost_wave(f, G, bc[w], io.s);
+        JCHK(hipGetLastError());
+        JCHK(hipEventSynchronize(io.ev));
+        double maxError;
+        unsigned long long bits = *io.hmax;
+        std::memcpy(&maxError, &bits, sizeof(double));
+        n += 1;
+        if (maxError < tol) break;
+        iterCount += 1;
+        if (iterCount > max_iter) {
+            status = 1;
+            break;
+        }
+        std::swap(F, G);
+    }
+#undef JCHK
+    *sweeps = n;
+    return status;
 }
 
 void ts_divergence(const FShape &fp, const FShape *fv, const Metrics &M, const double *const *V, double *out,
