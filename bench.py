@@ -224,8 +224,10 @@ def tgv_bench(args):
         o = po.OracleTS(po.ts_params(ind=(7, 7, 7), depth=6, smoother=po.SMOOTH_LEX), fast=True)
         o.init_tgv()
         c0 = time.perf_counter()
-        for _ in range(n_cpu):
+        for n in range(n_cpu):
             o.time_advance()
+            if (n + 1) % 10 == 0:
+                print(f"[tgv129] oracle (lexicographic) step {n + 1}", file=sys.stderr, flush=True)
         c1 = time.perf_counter()
         o.close()
         cpu = {"value": round((c1 - c0) * 1e3 / n_cpu, 1), "unit": "ms/step", "cores": 1, "kind": "port", "host": host_info(),
