@@ -394,7 +394,8 @@ def rbc_bench(args):
         "scaling": "strong", "vs_baseline": None, "dtype": "f64+f32+bf16", "data": "synthetic",
         "config": {"workload": f"Rayleigh-Benard {n[0]}x{n[1]}x{n[2]} nodes, PPN, tanh z beta=1, Ra=1e7, Pr=1, "
                                f"rbcType 2, dt={args.dt}, T perturbation 1e-3 (seed 0): the pressure solve of every "
-                               "time step to 1e-10 rel. residual; learned guess trained online over the warm-up steps",
+                               f"time step to 1e-10 rel. residual; learned guess trained online over the first "
+                               f"{args.train_steps} steps, then warm-up and timed steps from that state",
                    "nodes": list(n), "vcDepth": nz_ind - 1, "guess_history": args.history,
                    "learned_mixed": learned, "fp64_warm": out["fp64_warm"],
                    "mixed_refinement_warm": out["mixed_refinement_warm"], "gcr_warm": out["gcr_warm"],
@@ -430,7 +431,7 @@ def rbc_bench(args):
 
 
 def seq_bench(args):
-    """The headline: a sequence of 257^3 stretched-grid Poisson solves whose right-hand side
+    """Workload seq257: a sequence of 257^3 stretched-grid Poisson solves whose right-hand side
     evolves smoothly in time (the seeded Fourier modes of the single-solve workload with amplitudes
     a_m cos(w_m t + psi_m), dt 0.05), as a time stepper produces them.  Every solve runs to the same
     stop rule (relative residual 1e-10 on the projected right-hand side).

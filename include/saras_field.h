@@ -4,7 +4,7 @@
  *
  * These entry points stand for the methods a time step calls on its fields: derivative
  * (lib/field/derivative.h:78-84), vfield (lib/field/vfield.h:76-80), sfield (lib/field/sfield.h:73-76),
- * the boundary classes (lib/boundary/*.h) with mpidata::syncData, and the implicit Jacobi solves of
+ * the boundary classes (the headers of lib/boundary/) with mpidata::syncData, and the implicit Jacobi solves of
  * hydro_d3 / scalar_d3.  Every array is a device pointer the caller owns, in the reference's padded
  * layout: C order, k (z) fastest, logical index -1..n in each direction (lib/field/field.cc:90-91).
  * Core sizes follow the staggering of field.h:70-75 (vfield.cc:64-66):
@@ -54,7 +54,7 @@ typedef struct sr_fgrid_desc {
 #define SR_BC_NONE      3        /* the wall is left as it is: all six SR_BC_NONE is syncData alone */
                                  /* (vfield::syncData, sfield::syncData, plainsf::syncData)          */
 typedef struct sr_bc {
-    int kind[6];                    /* SR_BC_*; the CC or FC class of lib/boundary/*.cc follows    */
+    int kind[6];                    /* SR_BC_*; the CC or FC class of the files of lib/boundary/ follows */
                                     /* the field's staggering along the wall's direction           */
     double value[6];                /* Dirichlet wall values (dirichletCC sets the wall slice,     */
                                     /* dirichletFC sets 2 v - F at the image)                      */
@@ -113,7 +113,7 @@ int sr_gradient(sr_fgrid *g, const double *F, double *Gx, double *Gy, double *Gz
 /* boundary conditions of one field as imposeU/V/WBCs and imposeTBCs run them (hydro.cc:386-461,
    scalar.cc:240-255): mpidata::syncData with the rank its own neighbour along periodic x and y
    (mpidata.cc:278-306, faces then edges), the x walls, the y walls, then the top and the bottom
-   wall (lib/boundary/*.cc).  A periodic direction of the grid takes SR_BC_PERIODIC or SR_BC_NONE on
+   wall (the files of lib/boundary/).  A periodic direction of the grid takes SR_BC_PERIODIC or SR_BC_NONE on
    both walls, and only a periodic direction takes SR_BC_PERIODIC. */
 int sr_bc_fill(sr_fgrid *g, int loc, const sr_bc *bc, double *F);
 

@@ -387,21 +387,76 @@ __global__ void __launch_bounds__(256) k_lincomb(FineDims f, const double *__res
     if (part) block_sums<1>(v, part);
 }
 
-// out[c] = sum over blocks of part[b * w + c], c < w <= 16: 64 lanes per column
-__global__ void __launch_bounds__(1024) k_sumw_final(const double *__restrict__ part, int nb, int w,
+// out[c] = sum over blocks of part[b * W + c], c < W: thread t adds rows t, t + 1024, ... (four
+// rows loaded before any is added), then each column is added over the block in a fixed order
+// (wave shuffles, then the 16 wave sums by one thread)
+template <int W>
+__global__ void __launch_bounds__(1024) k_sumw_final(const double *__restrict__ part, int nb,
                                                      double *__restrict__ out) {
-    __shared__ double red[1024];
-    int c = threadIdx.x >> 6, l = threadIdx.x & 63;
-    double s = 0.0;
-    if (c < w)
-        for (int b = l; b < nb; b += 64) s += part[(long)b * w + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = 32; h > 0; h >>= 1) {
-        if (l < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
+    double s[W];
+#pragma unroll
+    for (int c = 0; c < W; c++) s[c] = 0.0;
+    for (int b0 = (int)threadIdx.x; b0 < nb; b0 += 4 * 1024) {
+        double v[4][W];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int b = b0 + u * 1024;
+#pragma unroll
+            for (int c = 0; c < W; c++) v[u][c] = b < nb ? part[(long)b * W + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int c = 0; c < W; c++) s[c] += v[u][c];
     }
-    if (l == 0 && c < w) out[c] = red[threadIdx.x];
+    __shared__ double red[W][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+        double x = s[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[c][wv] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < W) {
+        double a = 0.0;
+        for (int q = 0; q < 16; q++) a += red[threadIdx.x][q];
+        out[threadIdx.x] = a;
+    }
+}
+
+// Plane-chunk geometry over the padded array: block b covers in-plane entries
+// [256 (b % bpp), +256) of planes b / bpp, b / bpp + groups, ...; the in-plane split (j, k) is
+// fixed per thread, so the loop over planes needs no division.
+struct PlaneChunks {
+    unsigned n1, n2, planes, bpp, groups;
+    float inv2;
+};
+
+__device__ __forceinline__ unsigned fdiv_u(unsigned r, unsigned d, float inv) {
+    unsigned q = (unsigned)((float)r * inv);
+    int rem = (int)r - (int)(q * d);
+    q += (unsigned)(rem >= (int)d) - (unsigned)(rem < 0);
+    return q;
+}
+
+// k_wsum in plane chunks (in-plane entries below 2^24, where fdiv_u is exact)
+__global__ void __launch_bounds__(256) k_wsum_pc(FineDims f, PlaneChunks pc, const double *__restrict__ rhs,
+                                                 const double *__restrict__ wx, const double *__restrict__ wy,
+                                                 const double *__restrict__ wz, double *__restrict__ part) {
+    double v[1] = {0.0};
+    const unsigned chunk = blockIdx.x % pc.bpp, g = blockIdx.x / pc.bpp;
+    const unsigned r = chunk * 256u + threadIdx.x;
+    const unsigned jj = fdiv_u(r, pc.n2, pc.inv2), kk = r - jj * pc.n2;
+    const int j = (int)jj - 1, k = (int)kk - 1;
+    if (r < pc.n1 * pc.n2 && j >= 0 && j < f.ny && k >= 0 && k < f.nz) {
+        const double wjk = wy[j] * wz[k];
+        const long ps = (long)pc.n1 * pc.n2;
+#pragma unroll 4
+        for (unsigned ip = 1 + g; ip <= (unsigned)f.nx; ip += pc.groups) v[0] += wx[ip - 1] * wjk * rhs[ip * ps + r];
+    }
+    block_sums<1>(v, part);
 }
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
@@ -486,9 +541,23 @@ static int lin_fit(sr_guess *g, sr_mg *m, const double *rhs) {
     hipStream_t s = g->s;
     const FineDims f{g->fine[0], g->fine[1], g->fine[2]};
     if (singular_mg(m)) {
-        hipLaunchKernelGGL(k_wsum, dim3(kLinBlocks), dim3(256), 0, s, f, rhs, m->wvec[0], m->wvec[1], m->wvec[2],
-                           g->lpart);
-        hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kLinBlocks, 1, g->lsum);
+        PlaneChunks pc;
+        pc.n1 = (unsigned)f.ny + 2;
+        pc.n2 = (unsigned)f.nz + 2;
+        pc.planes = (unsigned)f.nx + 2;
+        pc.bpp = (pc.n1 * pc.n2 + 255u) / 256u;
+        pc.groups = std::max(1u, std::min((unsigned)f.nx, (unsigned)kLinBlocks / pc.bpp));
+        pc.inv2 = 1.0f / (float)pc.n2;
+        const unsigned nb = pc.bpp * pc.groups;
+        if ((long)pc.n1 * pc.n2 < (1L << 24) && nb <= (unsigned)kLinBlocks * 16u) {
+            hipLaunchKernelGGL(k_wsum_pc, dim3(nb), dim3(256), 0, s, f, pc, rhs, m->wvec[0], m->wvec[1], m->wvec[2],
+                               g->lpart);
+            hipLaunchKernelGGL(k_sumw_final<1>, dim3(1), dim3(1024), 0, s, g->lpart, (int)nb, g->lsum);
+        } else {
+            hipLaunchKernelGGL(k_wsum, dim3(kLinBlocks), dim3(256), 0, s, f, rhs, m->wvec[0], m->wvec[1],
+                               m->wvec[2], g->lpart);
+            hipLaunchKernelGGL(k_sumw_final<1>, dim3(1), dim3(1024), 0, s, g->lpart, kLinBlocks, g->lsum);
+        }
         hipLaunchKernelGGL(k_project_rhs, dim3(nblk(g->fine_pad)), dim3(256), 0, s, f, rhs, g->lsum, 1.0 / m->wsum,
                            g->fcur);
     } else {
@@ -496,7 +565,7 @@ static int lin_fit(sr_guess *g, sr_mg *m, const double *rhs) {
                            0.0, g->fcur);
     }
     hipLaunchKernelGGL(k_dots, dim3(kLinBlocks), dim3(256), 0, s, f, g->fcur, ptrs(g->hf, g->hn), g->hn, g->lpart);
-    hipLaunchKernelGGL(k_sumw_final, dim3(1), dim3(1024), 0, s, g->lpart, kLinBlocks, 9, g->lsum);
+    hipLaunchKernelGGL(k_sumw_final<9>, dim3(1), dim3(1024), 0, s, g->lpart, kLinBlocks, g->lsum);
     HIPCHK(hipMemcpyAsync(g->lsum_h, g->lsum, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));

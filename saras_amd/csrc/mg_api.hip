@@ -474,12 +474,14 @@ void sr::mark(sr_mg *m, hipEvent_t e) {
 // smooth(): ghosts, then `count` red-black sweeps (each updated node refreshes its face ghosts);
 // exact = 1 adds the closing imposeBC of poisson_d3.cc:181 (edge and corner ghosts included).
 void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact, hipEvent_t ea,
-                      hipEvent_t eb) {
+                      hipEvent_t eb, bool faces_ready) {
     Level &V = m->lv[l];
     Bc bc = pressure_bc(m, l, zeroBC);
     hipStream_t s = m->stream;
     if (m->d.gsSmooth) {
-        if (!zero_start) launch_fill_ghosts<double>(V.L, V.u, bc, s);
+        // the red-black sweeps read face ghosts only: a level whose face-ghost images are current
+        // (faces_ready) needs no fill
+        if (!zero_start && !faces_ready) launch_fill_ghosts<double>(V.L, V.u, bc, s);
         // a zero start without sweeps still starts from zero (poisson.cc clears the level)
         if (zero_start && count == 0) launch_fill<double>(V.u, lay_size(V.L), 0.0, s);
         if (ea) mark(m, ea);
@@ -554,9 +556,15 @@ void sr::vcycle_fp64(sr_mg *m) {
     // post-smoothing sweeps colour 0 first, which reads only colour-1 nodes: the correction of the
     // colour-0 nodes would be overwritten unread (Gauss-Seidel with at least one sweep)
     const int pcol = d.gsSmooth && d.postSmooth > 0 ? 1 : -1;
+    // below level 0 the prolongation writes the face-ghost images of what it updates (every ghost
+    // rule there is an image rule), so the post-smoothing starts without a ghost fill; level 0 keeps
+    // the separate fill (measured faster there)
     for (int l = bottom - 1; l >= 0; l--) {
-        launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream, pcol);
-        smooth_level(m, l, d.postSmooth, l == 0 ? 0 : 1, false, false);
+        const bool img = l > 0 && d.gsSmooth;
+        const Bc bc = pressure_bc(m, l, 1);
+        launch_prolong<double, double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u, m->lv[l].u, 1, m->stream, pcol,
+                                       img ? &bc : nullptr);
+        smooth_level(m, l, d.postSmooth, l == 0 ? 0 : 1, false, false, nullptr, nullptr, img);
     }
 }
 
@@ -571,13 +579,13 @@ Bc sr::error_bc(const sr_mg *m, int l, int zeroBC) {
 }
 
 static void smooth32(sr_mg *m, int l, int count, int zeroBC, bool zero_start, hipEvent_t ea = nullptr,
-                     hipEvent_t eb = nullptr) {
+                     hipEvent_t eb = nullptr, bool faces_ready = false) {
     Level &V = m->lv[l];
     Bc bc = error_bc(m, l, zeroBC);
     bool wall = bc.rule[0] == G_WALL || bc.rule[1] == G_WALL || bc.rule[2] == G_WALL;
     bool zs = zero_start && !wall;
     hipStream_t s = m->stream;
-    if (!zs) {
+    if (!zs && !(faces_ready && !zero_start)) {
         if (zero_start) launch_fill<float>(V.u32, lay_size(V.L), 0.0f, s);
         launch_fill_ghosts<float>(V.L, V.u32, bc, s);
     } else if (count == 0) {
@@ -624,9 +632,13 @@ void sr::vcycle_fp32(sr_mg *m) {
     }
     // colour 0 is swept first by the post-smoothing: its correction would be overwritten unread
     const int pcol = m->post32 > 0 ? 1 : -1;
+    // below level 0 the prolongation writes the face-ghost images of what it updates, so the
+    // post-smoothing starts without a ghost fill (as in the FP64 cycle)
     for (int l = bottom - 1; l >= 0; l--) {
-        launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s, pcol);
-        smooth32(m, l, m->post32, l == 0 ? 0 : 1, false);
+        const Bc bc = error_bc(m, l, 1);
+        launch_prolong<float, float>(m->lv[l].L, m->lv[l + 1].L, m->lv[l + 1].u32, m->lv[l].u32, 1, s, pcol,
+                                     l > 0 ? &bc : nullptr);
+        smooth32(m, l, m->post32, l == 0 ? 0 : 1, false, nullptr, nullptr, l > 0);
     }
 }
 

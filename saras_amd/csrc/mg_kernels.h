@@ -21,8 +21,10 @@ template <typename T>
 void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0 = false);
 template <typename T, typename TF>
 // color >= 0: that colour's nodes only (the up leg before red-black smoothing that sweeps the other
-// colour first); -1: every core node
-void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s, int color = -1);
+// colour first); -1: every core node.  img: the updated nodes also write their face-ghost images by *img
+// (periodic, mirror and antisymmetric rules; other rules' ghosts are left as they are)
+void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s, int color = -1,
+                    const Bc *img = nullptr);
 template <typename T> void launch_fill(T *a, long n, T v, hipStream_t s);
 template <typename T>
 void launch_add_scalar(const Lay &L, T *a, const double *dv, double scale, int core, hipStream_t s);
@@ -41,6 +43,13 @@ int norm_res_parts(const Lay &L);
 // fterms[1] = sum f^2 (core), fterms[3] = max |f| (every padded node); the right-hand side part of
 // computeError, reduced once per right-hand side
 template <typename T> void launch_f_terms(const Lay &L, const T *f, double *part, double *fterms, hipStream_t s);
+// q0 = weighted core sum of a reference padded array (out[0]), read in place
+void launch_wsum_ref(const Lay &L, const double *ref, const double *wx, const double *wy, const double *wz,
+                     double *part, double *out, hipStream_t s);
+// split-layout pack of a reference padded array with the shift scale * q[0] on core nodes, and the
+// right-hand side terms of computeError of the result into fterms
+void launch_pack_shift(const Lay &L, const double *ref, double *dst, const double *q, double scale, double *part,
+                       double *fterms, hipStream_t s);
 // out[0..3] = sum |Lu-f|^2, sum f^2 (core), max |Lu-f| (core), max |f| (every padded node);
 // out[1] and out[3] are copied from fterms
 template <typename T>
@@ -55,9 +64,10 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
 // coefficients from e = M r32 and the previous direction z (coef[0] alpha, coef[1] beta; part5 holds
 // 5 doubles per block of norm_res_parts), then the step x_out = x_in + alpha (e - beta z),
 // z_next = e - beta z (face-ghost images by bc and ebc); the FP64 residual of x_out is then
-// launch_outer_residual's
+// launch_outer_residual's.  q1r: e ends on a colour-1 red-black half sweep of level 0 for r32, so
+// L e = r32 on colour 1 and is read there instead of formed
 void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const float *cx,
-                     const float *cy, const float *cz, double *part5, double *coef, hipStream_t s);
+                     const float *cy, const float *cz, double *part5, double *coef, bool q1r, hipStream_t s);
 void launch_gcr_step(const Lay &L, const double *x_in, const float *e, const float *z, const double *coef,
                      double *x_out, float *z_next, const Bc &bc, const Bc &ebc, hipStream_t s);
 // exact coarsest solve u = P f with the dense inverse (transposed, nc x nc), ghosts filled by bc

@@ -312,9 +312,10 @@ __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T 
 // updates that colour's nodes only (geometry geo_sweep): a red-black sweep of the other colour
 // overwrites those nodes next without reading them, so their correction would be dead work.
 // ------------------------------------------------------------------------------------------------
-template <typename T, typename TF>
+// IMG: the updated node's face-ghost images by bc (image rules only) are written too
+template <typename T, typename TF, bool IMG>
 __global__ void __launch_bounds__(256) k_prolong(Lay F, Lay Cl, Geo G, const T *__restrict__ e, TF *__restrict__ u,
-                                                 int accumulate, int color) {
+                                                 int accumulate, int color, Bc bc) {
     unsigned ipu, rr;
     if (!geo_slot(G, xcd_block(), ipu, rr)) return;
     int ip = (int)ipu;
@@ -349,8 +350,9 @@ __global__ void __launch_bounds__(256) k_prolong(Lay F, Lay Cl, Geo G, const T *
         v = EE(0, 0, 0);
 #undef EE
     long o = lay_row(F, ip, jp, c) + h;
-    if (accumulate) u[o] = (TF)((TF)u[o] + (TF)v);
-    else u[o] = (TF)v;
+    const TF w = accumulate ? (TF)((TF)u[o] + (TF)v) : (TF)v;
+    u[o] = w;
+    if (IMG) write_images(F, u, ip, jp, kp, w, bc);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -536,6 +538,7 @@ __global__ void __launch_bounds__(256) k_norm_res(Lay L, Geo G, const T *__restr
 __global__ void __launch_bounds__(1024) k_norm_combine(const double *__restrict__ part, int nb,
                                                        const double *__restrict__ ft, double *__restrict__ out) {
     double s = 0.0, m = 0.0;
+#pragma unroll 8
     for (int b = (int)threadIdx.x; b < nb; b += 1024) {
         s += part[2 * (long)b];
         m = fmax(m, part[2 * (long)b + 1]);
@@ -584,6 +587,58 @@ __global__ void __launch_bounds__(256) k_f_terms(Lay L, Geo G, const T *__restri
     block_reduce4(0.0, s1, 0.0, m3, part);
 }
 
+// Weighted core sum q0 = sum w_x(i) w_y(j) w_z(k) ref of a reference padded array (the projection's
+// left-null-vector weight), read in place: work item li = (core plane p, 256-entry chunk of the
+// plane), the plane index uniform per block; per-block partials {q0, 0, 0, 0}.
+__global__ void __launch_bounds__(256) k_wsum_ref(Lay L, unsigned bpp, float inv2, const double *__restrict__ ref,
+                                                  const double *__restrict__ wx, const double *__restrict__ wy,
+                                                  const double *__restrict__ wz, double *__restrict__ part) {
+    const unsigned n1 = (unsigned)L.ny + 2, n2 = (unsigned)L.nz + 2, total = (unsigned)L.nx * bpp;
+    double s0 = 0.0;
+#pragma unroll 4
+    for (unsigned li = blockIdx.x; li < total; li += gridDim.x) {
+        const unsigned p = li / bpp;
+        const unsigned r = (li - p * bpp) * 256u + threadIdx.x;
+        const unsigned jj = fdivu(r, n2, inv2), kk = r - jj * n2;
+        if (r < n1 * n2 && jj >= 1 && jj <= (unsigned)L.ny && kk >= 1 && kk <= (unsigned)L.nz)
+            s0 += wx[p] * wy[jj - 1] * wz[kk - 1] * ref[(long)(p + 1) * (n1 * n2) + r];
+    }
+    block_reduce4(s0, 0.0, 0.0, 0.0, part);
+}
+
+// Every entry of the split layout from a reference padded array in one pass, with the projection's
+// shift: core nodes ref + scale q[0], ghosts and unused slots 0 (the state of a full pack followed
+// by a core-only shift), and the per-block partials {0, sum v^2 (core), 0, max |v| (core)} of the
+// right-hand side terms of computeError.  After the ghost fill of the right-hand side (periodic
+// images in x and y, z ghosts kept at 0) the maximum over every padded node equals the core one,
+// so these are the terms k_f_terms would reduce.
+__global__ void __launch_bounds__(256) k_pack_shift(Lay L, Geo G, const double *__restrict__ ref,
+                                                    double *__restrict__ dst, const double *__restrict__ q,
+                                                    double scale, double *__restrict__ part) {
+    const double sh = scale * q[0];
+    double s1 = 0.0, m3 = 0.0;
+    unsigned lb0, lbs, lbe;
+    xcd_range(G.total_blocks, lb0, lbs, lbe);
+    for (unsigned lb = lb0; lb < lbe; lb += lbs) {
+        unsigned ipu, r;
+        if (!geo_slot(G, lb, ipu, r)) continue;
+        const unsigned jpu = fdivu(r, G.d1, G.inv1);
+        const unsigned r2 = r - jpu * G.d1;
+        const int c = r2 >= (unsigned)L.khs ? 1 : 0;
+        const int h = (int)r2 - c * L.khs;
+        const int ip = (int)ipu, jp = (int)jpu;
+        const int kp = 2 * h + ((c + 1 + ip + jp) & 1);
+        double v = 0.0;
+        if (h < L.kh && ip >= 1 && ip <= L.nx && jp >= 1 && jp <= L.ny && kp >= 1 && kp <= L.nz) {
+            v = ref[((long)ip * (L.ny + 2) + jp) * (L.nz + 2) + kp] + sh;
+            s1 += v * v;
+            m3 = fmax(m3, fabs(v));
+        }
+        dst[(long)ip * G.per_plane + r] = v;
+    }
+    block_reduce4(0.0, s1, 0.0, m3, part);
+}
+
 // weighted sums over the reference right-hand side: q0 = sum w_i w_j w_k f (core), q1 = sum f over
 // every padded node
 template <typename T>
@@ -613,12 +668,23 @@ __global__ void __launch_bounds__(256) k_rhs_sums(Lay L, Geo G, const T *__restr
 
 __global__ void __launch_bounds__(256) k_reduce_final(const double *__restrict__ part, int nparts, double *__restrict__ out) {
     double s0 = 0, s1 = 0, m2 = 0, m3 = 0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-        const double *p = part + (long)i * NQ;
-        s0 += p[0];
-        s1 += p[1];
-        m2 = fmax(m2, p[2]);
-        m3 = fmax(m3, p[3]);
+    // eight partials per thread loaded before any is added (the loads overlap); the sums run in
+    // partial order as in a plain strided loop.  m2 and m3 are maxima of absolute values (>= 0).
+    const int step = (int)blockDim.x;
+    for (int i0 = threadIdx.x; i0 < nparts; i0 += 8 * step) {
+        double4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u * step;
+            v[u] = i < nparts ? *reinterpret_cast<const double4 *>(part + (long)i * NQ) : make_double4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            s0 += v[u].x;
+            s1 += v[u].y;
+            m2 = fmax(m2, v[u].z);
+            m3 = fmax(m3, v[u].w);
+        }
     }
     // single block: its one "partial" is the result
     block_reduce4(s0, s1, m2, m3, out);
@@ -827,6 +893,9 @@ __device__ inline float lap32(const Lay &L, const float *__restrict__ a, long rc
            Z[0] * a[ro + h + kb] + Z[1] * a[ro + h - 1 + kb] - diag3<float>(X, Y, Z) * a[rc + h];
 }
 
+// Q1R: e ends on a colour-1 red-black half sweep of level 0 for the right-hand side r32, which
+// solves the colour-1 equations: (L e) = r32 there (to FP32 rounding), so q is read, not formed.
+template <bool Q1R>
 __global__ void __launch_bounds__(256) k_gcr_dots(Lay L, Geo G, const float *__restrict__ e,
                                                   const float *__restrict__ z, const float *__restrict__ r32,
                                                   const float *__restrict__ cx, const float *__restrict__ cy,
@@ -835,9 +904,9 @@ __global__ void __launch_bounds__(256) k_gcr_dots(Lay L, Geo G, const float *__r
     core4_visit(L, G, [&](int ip, int jp, int c, int h, int kb, int kp) {
         long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
         const float *X = cx + 4 * ip, *Y = cy + 4 * jp, *Z = cz + 4 * kp;
-        double q = (double)lap32(L, e, rc, ro, h, kb, X, Y, Z);
-        double p = (double)lap32(L, z, rc, ro, h, kb, X, Y, Z);
         double r = (double)r32[rc + h];
+        double q = Q1R && c == 1 ? r : (double)lap32(L, e, rc, ro, h, kb, X, Y, Z);
+        double p = (double)lap32(L, z, rc, ro, h, kb, X, Y, Z);
         s[0] += r * q;
         s[1] += q * q;
         s[2] += q * p;
@@ -863,6 +932,7 @@ __global__ void __launch_bounds__(256) k_gcr_dots(Lay L, Geo G, const float *__r
 // previous direction) the step falls back to the minimal-residual step along e.
 __global__ void __launch_bounds__(1024) k_gcr_coef(const double *__restrict__ part, int nb, double *__restrict__ coef) {
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
     for (int b = (int)threadIdx.x; b < nb; b += 1024)
 #pragma unroll
         for (int i = 0; i < 5; i++) s[i] += part[5 * (long)b + i];
@@ -1497,10 +1567,15 @@ void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t
 }
 
 template <typename T, typename TF>
-void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s, int color) {
+void launch_prolong(const Lay &F, const Lay &Cl, const T *e, TF *u, int accumulate, hipStream_t s, int color,
+                    const Bc *img) {
     Geo G = color >= 0 ? geo_sweep(F) : geo_core(F);
-    hipLaunchKernelGGL((k_prolong<T, TF>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, e, u, accumulate,
-                       color);
+    if (img)
+        hipLaunchKernelGGL((k_prolong<T, TF, true>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, e, u,
+                           accumulate, color, *img);
+    else
+        hipLaunchKernelGGL((k_prolong<T, TF, false>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, e, u,
+                           accumulate, color, Bc{});
 }
 
 template <typename T>
@@ -1585,10 +1660,13 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
 }
 
 void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const float *cx,
-                     const float *cy, const float *cz, double *part5, double *coef, hipStream_t s) {
+                     const float *cy, const float *cz, double *part5, double *coef, bool q1r, hipStream_t s) {
     Geo G = geo_core4(L);
     int nb = (int)G.total_blocks;
-    hipLaunchKernelGGL(k_gcr_dots, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);
+    if (q1r)
+        hipLaunchKernelGGL(k_gcr_dots<true>, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);
+    else
+        hipLaunchKernelGGL(k_gcr_dots<false>, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);
     hipLaunchKernelGGL(k_gcr_coef, dim3(1), dim3(1024), 0, s, part5, nb, coef);
 }
 
@@ -1615,6 +1693,24 @@ void launch_f_terms(const Lay &L, const T *f, double *part, double *fterms, hipS
     Geo G = geo_all(L);
     int nb = (int)reduce_grid(G);
     hipLaunchKernelGGL(k_f_terms<T>, dim3(nb), dim3(256), 0, s, L, G, f, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, fterms);
+}
+
+void launch_wsum_ref(const Lay &L, const double *ref, const double *wx, const double *wy, const double *wz,
+                     double *part, double *out, hipStream_t s) {
+    const unsigned n2 = (unsigned)L.nz + 2;
+    const unsigned bpp = ((unsigned)(L.ny + 2) * n2 + 255u) / 256u;
+    const unsigned total = (unsigned)L.nx * bpp;
+    const unsigned nb = total < REDUCE_BLOCKS ? total : REDUCE_BLOCKS;
+    hipLaunchKernelGGL(k_wsum_ref, dim3(nb), dim3(256), 0, s, L, bpp, 1.0f / (float)n2, ref, wx, wy, wz, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, (int)nb, out);
+}
+
+void launch_pack_shift(const Lay &L, const double *ref, double *dst, const double *q, double scale, double *part,
+                       double *fterms, hipStream_t s) {
+    Geo G = geo_all(L);
+    int nb = (int)reduce_grid(G);
+    hipLaunchKernelGGL(k_pack_shift, dim3(nb), dim3(256), 0, s, L, G, ref, dst, q, scale, part);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, nb, fterms);
 }
 
@@ -1704,9 +1800,9 @@ template void launch_residual<float, float>(const Lay &, const float *, const fl
                                             const float *, const float *, hipStream_t, int);
 template void launch_residual<double, float>(const Lay &, const double *, const double *, float *,
                                              const double *, const double *, const double *, hipStream_t, int);
-template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t, int);
-template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t, int);
-template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t, int);
+template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t, int, const Bc *);
+template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t, int, const Bc *);
+template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t, int, const Bc *);
 template void launch_pack<double, double>(const Lay &, const double *, double *, int, hipStream_t);
 template void launch_pack<double, float>(const Lay &, const double *, float *, int, hipStream_t);
 template void launch_pack_full<double, double>(const Lay &, const double *, double *, hipStream_t);

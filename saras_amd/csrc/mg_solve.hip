@@ -67,13 +67,17 @@ static int stage_alloc(sr_mg *m) {
 static void load_level0(sr_mg *m, const double *drhs, const double *dp) {
     Level &V = m->lv[0];
     hipStream_t s = m->stream;
-    launch_pack_full<double, double>(V.L, drhs, V.f, s);
-    launch_pack_full<double, double>(V.L, dp, V.u, s);
     bool proj = m->d.semantics == SR_SEM_PROJECTED && is_singular(m);
     if (proj) {
-        launch_rhs_sums<double>(V.L, V.f, m->wvec[0], m->wvec[1], m->wvec[2], m->part, m->dres, s);
-        launch_add_scalar<double>(V.L, V.f, m->dres, -1.0 / m->wsum, 1, s);
+        // the weighted sum read from the caller's array, then one pass that packs it with the shift
+        // and reduces the computeError terms of the result (pads_rules: x/y periodic images and z
+        // ghosts kept at 0, so the padded maximum is the core one)
+        launch_wsum_ref(V.L, drhs, m->wvec[0], m->wvec[1], m->wvec[2], m->part, m->dres, s);
+        launch_pack_shift(V.L, drhs, V.f, m->dres, -1.0 / m->wsum, m->part, m->fterms, s);
+    } else {
+        launch_pack_full<double, double>(V.L, drhs, V.f, s);
     }
+    launch_pack_full<double, double>(V.L, dp, V.u, s);
     launch_fill_ghosts<double>(V.L, V.f, pads_rules(m), s);
     launch_fill_ghosts<double>(V.L, V.u, pads_rules(m), s);
     if (m->d.semantics == SR_SEM_REFERENCE && !m->d.testPoisson) {
@@ -81,7 +85,7 @@ static void load_level0(sr_mg *m, const double *drhs, const double *dp) {
         launch_rhs_sums<double>(V.L, V.f, nullptr, nullptr, nullptr, m->part, m->dres, s);
         launch_add_scalar<double>(V.L, V.f, m->dres + 1, -1.0 / npad, 0, s);
     }
-    f_terms_level0(m);
+    if (!proj) f_terms_level0(m);
 }
 
 static int norm_order(const sr_mg *m) {
@@ -277,7 +281,11 @@ static void gcr_body(sr_mg *m, int cur) {
     const float *z = m->zdir[cur];
     float *z_next = m->zdir[cur ^ 1];
     vcycle_fp32(m);
-    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c32[0], V.c32[1], V.c32[2], m->gpart, m->gcoef, s);
+    // the FP32 cycle ends on level 0's post-smoothing (a colour-1 half sweep last) when level 0 is
+    // above the bottom of the cycle and has post-smoothing sweeps
+    const int Lc = m->coarse_enable ? m->coarse_first32 : (1 << 20);
+    const bool q1r = std::min(Lc, m->depth) >= 1 && m->post32 > 0;
+    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c32[0], V.c32[1], V.c32[2], m->gpart, m->gcoef, q1r, s);
     launch_gcr_step(V.L, x_in, V.u32, z, m->gcoef, x_out, z_next, pressure_bc(m, 0, 0), error_bc(m, 0, 0), s);
     launch_outer_residual(V.L, x_out, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
                           pressure_bc(m, 0, 0), m->part2, m->fterms, m->dres, s);

@@ -86,8 +86,10 @@ Bc pressure_bc(const sr_mg *m, int level, int zeroBC);
 Bc tmp_bc(const sr_mg *m);
 // ghost rules of a correction (pressure rules with homogeneous walls)
 Bc error_bc(const sr_mg *m, int l, int zeroBC);
+// faces_ready: the level's face-ghost images are current (written by the prolongation), so the
+// red-black sweeps start without a ghost fill
 void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
-                  hipEvent_t ea = nullptr, hipEvent_t eb = nullptr);
+                  hipEvent_t ea = nullptr, hipEvent_t eb = nullptr, bool faces_ready = false);
 void residual_level(sr_mg *m, int l, int color = -1);
 void coarsest_level(sr_mg *m, int l, int zeroBC);
 void vcycle_fp64(sr_mg *m);

@@ -273,6 +273,7 @@ __global__ void __launch_bounds__(256) k_jacobi_vres(FShape f, Metrics M, Bulk B
 __global__ void __launch_bounds__(1024) k_jacobi_res_final(const double *__restrict__ part, int n,
                                                            unsigned long long *__restrict__ out) {
     double m = 0.0;
+#pragma unroll 8
     for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
     __shared__ double s[16];
 #pragma unroll
@@ -499,6 +500,7 @@ __global__ void __launch_bounds__(256) k_energy(FShape fp, FShape fx, FShape fy,
 
 __global__ void __launch_bounds__(256) k_energy_final(const double *__restrict__ part, int n, double *__restrict__ out) {
     double a = 0, b = 0;
+#pragma unroll 8
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         a += part[2 * (long)q];
         b += part[2 * (long)q + 1];
@@ -542,6 +544,7 @@ __global__ void __launch_bounds__(256) k_max_partial(FShape fp, const double *__
 
 __global__ void __launch_bounds__(256) k_max_final(const double *__restrict__ part, int n, double *__restrict__ out) {
     double m = -1.0e300;
+#pragma unroll 8
     for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
     __shared__ double s[4];
 #pragma unroll
