@@ -354,6 +354,10 @@ static int create_impl(const sr_mg_desc *d, sr_mg *m) {
     HIPCHK(hipMalloc((void **)&m->part, sizeof(double) * 4 * (size_t)nparts));
     HIPCHK(hipMalloc((void **)&m->part2, sizeof(double) * 2 * (size_t)norm_res_parts(F)));
     HIPCHK(hipMalloc((void **)&m->fterms, sizeof(double) * 4));
+    HIPCHK(hipMalloc((void **)&m->fstage, sizeof(double) * FINAL_STAGE));
+    HIPCHK(hipMalloc((void **)&m->fcount, sizeof(unsigned)));
+    HIPCHK(hipMemsetAsync(m->fcount, 0, sizeof(unsigned), m->stream));
+    m->fb_coef = FinalBufs{m->fstage, m->fcount};
     launch_fill<double>(m->fterms, 4, 0.0, m->stream);
     HIPCHK(hipMalloc((void **)&m->dres, sizeof(double) * 16));
     HIPCHK(hipMalloc((void **)&m->diters, sizeof(int) * 4));
@@ -398,7 +402,7 @@ extern "C" int sr_mg_destroy(sr_mg *m) {
     }
     for (int q = 0; q < 6; q++) (void)hipFree(m->wall[q]);
     for (int a = 0; a < 3; a++) (void)hipFree(m->wvec[a]);
-    (void)hipFree(m->part); (void)hipFree(m->part2); (void)hipFree(m->fterms); (void)hipFree(m->dres); (void)hipFree(m->diters); (void)hipFree(m->stage);
+    (void)hipFree(m->part); (void)hipFree(m->part2); (void)hipFree(m->fterms); (void)hipFree(m->fstage); (void)hipFree(m->fcount); (void)hipFree(m->dres); (void)hipFree(m->diters); (void)hipFree(m->stage);
     (void)hipFree(m->stage2); (void)hipFree(m->r32);
     if (m->hres) (void)hipHostFree(m->hres);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -503,10 +507,18 @@ void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, b
     if (exact || !m->d.gsSmooth) launch_fill_ghosts<double>(V.L, m->lv[l].u, bc, s);
 }
 
-void sr::residual_level(sr_mg *m, int l, int color) {
+// every rule of b writes images (no kept or zero ghosts)
+static bool image_rules(const Bc &b) { return b.rule[0] <= G_ANTI && b.rule[1] <= G_ANTI && b.rule[2] <= G_ANTI; }
+
+void sr::residual_level(sr_mg *m, int l, int color, bool images) {
     Level &V = m->lv[l];
-    launch_residual<double, double>(V.L, V.u, V.f, V.t, V.c64[0], V.c64[1], V.c64[2], m->stream, color);
-    launch_fill_ghosts<double>(V.L, V.t, tmp_bc(m), m->stream);
+    const Bc tb = tmp_bc(m);
+    // with colour 0 only (read by the red-black restriction: coincident node and edge points) and
+    // image rules, the residual kernel writes the face and edge images itself
+    const bool img = images && color == 0 && image_rules(tb);
+    launch_residual<double, double>(V.L, V.u, V.f, V.t, V.c64[0], V.c64[1], V.c64[2], m->stream, color,
+                                    img ? &tb : nullptr);
+    if (!img) launch_fill_ghosts<double>(V.L, V.t, tb, m->stream);
 }
 
 void sr::coarsest_level(sr_mg *m, int l, int zeroBC) {
@@ -538,7 +550,7 @@ void sr::vcycle_fp64(sr_mg *m) {
     // colour 0 is computed and restricted
     const bool red0 = d.gsSmooth && d.preSmooth > 0;
     for (int l = 0; l < bottom; l++) {
-        residual_level(m, l, red0 ? 0 : -1);
+        residual_level(m, l, red0 ? 0 : -1, l > 0);
         launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream, red0);
         if (l + 1 == Lc) {
             launch_coarse_cycle<double>(m->cdesc64, m->clds64, m->lv[Lc].u, m->lv[Lc].f, m->diters, m->stream);
@@ -610,8 +622,12 @@ void sr::vcycle_fp32(sr_mg *m) {
     const bool red0 = m->pre32 > 0;
     for (int l = 0; l < bottom; l++) {
         Level &V = m->lv[l];
-        launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s, red0 ? 0 : -1);
-        launch_fill_ghosts<float>(V.L, V.t32, tmp_bc(m), s);
+        // below level 0 the colour-0 residual writes its face and edge images (image rules only)
+        const Bc tb = tmp_bc(m);
+        const bool img = l > 0 && red0 && image_rules(tb);
+        launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s, red0 ? 0 : -1,
+                                      img ? &tb : nullptr);
+        if (!img) launch_fill_ghosts<float>(V.L, V.t32, tb, s);
         launch_restrict<float>(V.L, m->lv[l + 1].L, V.t32, m->lv[l + 1].f32, s, red0);
         if (l + 1 == Lc) {
             launch_coarse_cycle<float>(m->cdesc32, m->clds32, m->lv[Lc].u32, m->lv[Lc].f32, m->diters, s);

@@ -13,8 +13,10 @@ template <typename T>
 void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, const T *cz, int color,
                      const Bc &bc, bool zero_start, hipStream_t s);
 template <typename T, typename TO>
+// img: each computed node also writes its face and edge ghost images by *img (image rules only;
+// corner ghosts are not written)
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
-                     hipStream_t s, int color = -1);
+                     hipStream_t s, int color = -1, const Bc *img = nullptr);
 // red0: t is zero on colour 1 and only its colour-0 entries are read (residual after a red-black
 // sweep ending on colour 1, computed with color = 0)
 template <typename T>
@@ -65,9 +67,11 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
 // 5 doubles per block of norm_res_parts), then the step x_out = x_in + alpha (e - beta z),
 // z_next = e - beta z (face-ghost images by bc and ebc); the FP64 residual of x_out is then
 // launch_outer_residual's.  q1r: e ends on a colour-1 red-black half sweep of level 0 for r32, so
-// L e = r32 on colour 1 and is read there instead of formed
+// L e = r32 on colour 1 and is read there instead of formed.  fb: the coefficients come from the
+// two-stage final over several workgroups (k_gcr_coef_mb) instead of one workgroup
 void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const float *cx,
-                     const float *cy, const float *cz, double *part5, double *coef, bool q1r, hipStream_t s);
+                     const float *cy, const float *cz, double *part5, double *coef, bool q1r, hipStream_t s,
+                     const FinalBufs *fb = nullptr);
 void launch_gcr_step(const Lay &L, const double *x_in, const float *e, const float *z, const double *coef,
                      double *x_out, float *z_next, const Bc &bc, const Bc &ebc, hipStream_t s);
 // exact coarsest solve u = P f with the dense inverse (transposed, nc x nc), ghosts filled by bc

@@ -195,6 +195,44 @@ __device__ inline void write_images(const Lay &L, T *u, int ip, int jp, int kp, 
     }
 }
 
+// Face and edge ghost images of node (ip, jp, kp) with value v, image rules only: along each
+// direction the ghosts whose source is this node, then every pair of directions (the composed image
+// of the x-then-y-then-z fill when every rule is an image rule).  Corner ghosts are not written: the
+// red-black restriction reads the coincident node and the edge points, at most two ghost coordinates.
+template <typename T>
+__device__ inline void write_images_fe(const Lay &L, T *a, int ip, int jp, int kp, T v, const Bc &bc) {
+    const int n[3] = {L.nx, L.ny, L.nz};
+    const int p[3] = {ip, jp, kp};
+    int g[3][2] = {{0, 0}, {0, 0}, {0, 0}}, cnt[3] = {0, 0, 0};
+    T w[3] = {T(1), T(1), T(1)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int r = bc.rule[d];
+        if (r > G_ANTI) continue;
+        w[d] = r == G_ANTI ? T(-1) : T(1);
+        if (p[d] == ghost_src(r, 0, n[d])) g[d][cnt[d]++] = 0;
+        if (p[d] == ghost_src(r, n[d] + 1, n[d])) g[d][cnt[d]++] = n[d] + 1;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        for (int t = 0; t < cnt[d]; ++t) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[d] = g[d][t];
+            a[lay_idx(L, q[0], q[1], q[2])] = w[d] * v;
+        }
+#pragma unroll
+    for (int d1 = 0; d1 < 2; ++d1)
+#pragma unroll
+        for (int d2 = d1 + 1; d2 < 3; ++d2)
+            for (int t1 = 0; t1 < cnt[d1]; ++t1)
+                for (int t2 = 0; t2 < cnt[d2]; ++t2) {
+                    int q[3] = {p[0], p[1], p[2]};
+                    q[d1] = g[d1][t1];
+                    q[d2] = g[d2][t2];
+                    a[lay_idx(L, q[0], q[1], q[2])] = w[d1] * w[d2] * v;
+                }
+}
+
 // ZERO_START: every opposite-colour value is known to be zero (first red half sweep after the level
 // was cleared), so the update reduces to -f / sum lc and no u is read.
 template <typename T, bool ZERO_START>
@@ -248,10 +286,12 @@ __device__ inline T apply_lap(const Lay &L, const T *__restrict__ u, long row_c,
 
 // color >= 0: that colour's nodes only (geometry geo_sweep), the other colour's entries of t_out
 // are left as they are
-template <typename T, typename TO>
+// IMG: the node's face and edge ghost images by bc are written too (write_images_fe)
+template <typename T, typename TO, bool IMG>
 __global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restrict__ u, const T *__restrict__ f,
                                                   TO *__restrict__ t_out, const T *__restrict__ cx,
-                                                  const T *__restrict__ cy, const T *__restrict__ cz, int color) {
+                                                  const T *__restrict__ cy, const T *__restrict__ cz, int color,
+                                                  Bc bc) {
     unsigned ipu, rr;
     if (!geo_slot(G, xcd_block(), ipu, rr)) return;
     int ip = (int)ipu;
@@ -265,7 +305,9 @@ __global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restr
     if (kp < 1 || kp > L.nz) return;
     long rc = lay_row(L, ip, jp, c), ro = lay_row(L, ip, jp, c ^ 1);
     T lu = apply_lap<T, TO>(L, u, rc, ro, h, kb, cx + 4 * ip, cy + 4 * jp, cz + 4 * kp);
-    t_out[rc + h] = (TO)(f[rc + h] - lu);
+    const TO r = (TO)(f[rc + h] - lu);
+    t_out[rc + h] = r;
+    if (IMG) write_images_fe(L, t_out, ip, jp, kp, r, bc);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -563,6 +605,26 @@ __global__ void __launch_bounds__(1024) k_norm_combine(const double *__restrict_
         out[3] = ft[3];
     }
 }
+
+// Two-stage final reductions (k_gcr_coef_mb): FINAL_BLOCKS workgroups each reduce a contiguous
+// share of the partials in a fixed order into a stage; the workgroup that finishes last (a
+// device-scope counter, which it resets) combines the stages in order, so the result does not
+// depend on the order in which the workgroups finish.
+constexpr int FINAL_BLOCKS = 16;
+static_assert(FINAL_BLOCKS * 5 <= FINAL_STAGE, "stage buffer of the two-stage finals");
+
+__device__ inline bool last_block_done(unsigned *counter) {
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
 
 // sum f^2 over the core and max |f| over every padded node
 template <typename T>
@@ -930,6 +992,20 @@ __global__ void __launch_bounds__(256) k_gcr_dots(Lay L, Geo G, const float *__r
 // alpha, beta of the step from the block sums; coef = {alpha, beta, (r q), (q q), (q p), (r p), (p p)}.
 // When the orthogonalised direction has lost most of its norm to cancellation (or there is no
 // previous direction) the step falls back to the minimal-residual step along e.
+__device__ inline void gcr_coef_from(const double (&t)[5], double *coef) {
+    double rq = t[0], qq = t[1], qp = t[2], rp = t[3], pp = t[4];
+    double beta = pp > 0.0 ? qp / pp : 0.0;
+    double qn = qq - 2.0 * beta * qp + beta * beta * pp;
+    if (!(qn >= 1e-3 * qq)) {
+        beta = 0.0;
+        qn = qq;
+    }
+    double alpha = qn > 0.0 ? (rq - beta * rp) / qn : 0.0;
+    coef[0] = alpha;
+    coef[1] = beta;
+    for (int i = 0; i < 5; i++) coef[2 + i] = t[i];
+}
+
 __global__ void __launch_bounds__(1024) k_gcr_coef(const double *__restrict__ part, int nb, double *__restrict__ coef) {
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
@@ -947,17 +1023,37 @@ __global__ void __launch_bounds__(1024) k_gcr_coef(const double *__restrict__ pa
         double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int w = 0; w < 16; w++)
             for (int i = 0; i < 5; i++) t[i] += sh[w][i];
-        double rq = t[0], qq = t[1], qp = t[2], rp = t[3], pp = t[4];
-        double beta = pp > 0.0 ? qp / pp : 0.0;
-        double qn = qq - 2.0 * beta * qp + beta * beta * pp;
-        if (!(qn >= 1e-3 * qq)) {
-            beta = 0.0;
-            qn = qq;
-        }
-        double alpha = qn > 0.0 ? (rq - beta * rp) / qn : 0.0;
-        coef[0] = alpha;
-        coef[1] = beta;
-        for (int i = 0; i < 5; i++) coef[2 + i] = t[i];
+        gcr_coef_from(t, coef);
+    }
+}
+
+// k_gcr_coef over FINAL_BLOCKS workgroups (two-stage form, see last_block_done)
+__global__ void __launch_bounds__(256) k_gcr_coef_mb(const double *__restrict__ part, int nb,
+                                                     double *__restrict__ coef, double *__restrict__ stage,
+                                                     unsigned *__restrict__ counter) {
+    const int per = (nb + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lo = (int)blockIdx.x * per, hi = min(nb, lo + per);
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int b = lo + (int)threadIdx.x; b < hi; b += 256)
+#pragma unroll
+        for (int i = 0; i < 5; i++) s[i] += part[5 * (long)b + i];
+    __shared__ double sh[4][5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) s[i] = wave_sum(s[i]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 5; i++) sh[threadIdx.x >> 6][i] = s[i];
+    __syncthreads();
+    if (threadIdx.x < 5) stage[5 * blockIdx.x + threadIdx.x] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) +
+                                                               (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+    if (!last_block_done(counter)) return;
+    if (threadIdx.x == 0) {
+        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < (int)gridDim.x; q++)
+            for (int i = 0; i < 5; i++) t[i] += stage[5 * q + i];
+        gcr_coef_from(t, coef);
+        atomicExch(counter, 0u);
     }
 }
 
@@ -1552,9 +1648,14 @@ void launch_rb_sweep(const Lay &L, T *u, const T *f, const T *cx, const T *cy, c
 
 template <typename T, typename TO>
 void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, const T *cy, const T *cz,
-                     hipStream_t s, int color) {
+                     hipStream_t s, int color, const Bc *img) {
     Geo G = color >= 0 ? geo_sweep(L) : geo_core(L);
-    hipLaunchKernelGGL((k_residual<T, TO>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy, cz, color);
+    if (img)
+        hipLaunchKernelGGL((k_residual<T, TO, true>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy,
+                           cz, color, *img);
+    else
+        hipLaunchKernelGGL((k_residual<T, TO, false>), dim3(G.total_blocks), dim3(256), 0, s, L, G, u, f, t, cx, cy,
+                           cz, color, Bc{});
 }
 
 template <typename T>
@@ -1660,14 +1761,19 @@ void launch_outer_residual(const Lay &L, const double *u_in, const float *e, con
 }
 
 void launch_gcr_dots(const Lay &L, const float *e, const float *z, const float *r32, const float *cx,
-                     const float *cy, const float *cz, double *part5, double *coef, bool q1r, hipStream_t s) {
+                     const float *cy, const float *cz, double *part5, double *coef, bool q1r, hipStream_t s,
+                     const FinalBufs *fb) {
     Geo G = geo_core4(L);
     int nb = (int)G.total_blocks;
     if (q1r)
         hipLaunchKernelGGL(k_gcr_dots<true>, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);
     else
         hipLaunchKernelGGL(k_gcr_dots<false>, dim3(nb), dim3(256), 0, s, L, G, e, z, r32, cx, cy, cz, part5);
-    hipLaunchKernelGGL(k_gcr_coef, dim3(1), dim3(1024), 0, s, part5, nb, coef);
+    if (fb)
+        hipLaunchKernelGGL(k_gcr_coef_mb, dim3(FINAL_BLOCKS), dim3(256), 0, s, part5, nb, coef, fb->stage,
+                           fb->counter);
+    else
+        hipLaunchKernelGGL(k_gcr_coef, dim3(1), dim3(1024), 0, s, part5, nb, coef);
 }
 
 void launch_gcr_step(const Lay &L, const double *x_in, const float *e, const float *z, const double *coef,
@@ -1795,11 +1901,11 @@ template int coarse_cycle_attr<float>(size_t);
 #undef SR_INST
 
 template void launch_residual<double, double>(const Lay &, const double *, const double *, double *,
-                                              const double *, const double *, const double *, hipStream_t, int);
+                                              const double *, const double *, const double *, hipStream_t, int, const Bc *);
 template void launch_residual<float, float>(const Lay &, const float *, const float *, float *, const float *,
-                                            const float *, const float *, hipStream_t, int);
+                                            const float *, const float *, hipStream_t, int, const Bc *);
 template void launch_residual<double, float>(const Lay &, const double *, const double *, float *,
-                                             const double *, const double *, const double *, hipStream_t, int);
+                                             const double *, const double *, const double *, hipStream_t, int, const Bc *);
 template void launch_prolong<double, double>(const Lay &, const Lay &, const double *, double *, int, hipStream_t, int, const Bc *);
 template void launch_prolong<float, float>(const Lay &, const Lay &, const float *, float *, int, hipStream_t, int, const Bc *);
 template void launch_prolong<float, double>(const Lay &, const Lay &, const float *, double *, int, hipStream_t, int, const Bc *);

@@ -78,4 +78,13 @@ struct CoarseDesc {
     int tap_max;        // coarse levels of at most this many core nodes use it
 };
 
+// Buffers of a two-stage final reduction (k_gcr_coef_mb): FINAL_STAGE doubles
+// of per-workgroup sums and a device-scope counter that is 0 between launches.  nullptr in a
+// launcher selects the single-workgroup final.
+struct FinalBufs {
+    double *stage;
+    unsigned *counter;
+};
+constexpr int FINAL_STAGE = 16 * 8;
+
 }  // namespace sr

@@ -285,7 +285,8 @@ static void gcr_body(sr_mg *m, int cur) {
     // above the bottom of the cycle and has post-smoothing sweeps
     const int Lc = m->coarse_enable ? m->coarse_first32 : (1 << 20);
     const bool q1r = std::min(Lc, m->depth) >= 1 && m->post32 > 0;
-    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c32[0], V.c32[1], V.c32[2], m->gpart, m->gcoef, q1r, s);
+    launch_gcr_dots(V.L, V.u32, z, V.f32, V.c32[0], V.c32[1], V.c32[2], m->gpart, m->gcoef, q1r, s,
+                    &m->fb_coef);
     launch_gcr_step(V.L, x_in, V.u32, z, m->gcoef, x_out, z_next, pressure_bc(m, 0, 0), error_bc(m, 0, 0), s);
     launch_outer_residual(V.L, x_out, nullptr, V.f, nullptr, V.f32, V.c64[0], V.c64[1], V.c64[2],
                           pressure_bc(m, 0, 0), m->part2, m->fterms, m->dres, s);

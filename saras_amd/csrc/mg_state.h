@@ -46,6 +46,9 @@ struct sr_mg {
     double *part = nullptr;      // reduction partials
     double *part2 = nullptr;     // per-block (sum, max) partials of the level-0 residual norm
     double *fterms = nullptr;    // right-hand side terms of computeError (device, 4 doubles)
+    double *fstage = nullptr;    // stage of the GCR coefficients' two-stage final (FINAL_STAGE doubles)
+    unsigned *fcount = nullptr;  // its counter (zero between launches)
+    sr::FinalBufs fb_coef{};
     double *dres = nullptr;      // reduced quantities (device)
     double *hres = nullptr;      // pinned host mirror
     int *diters = nullptr;
@@ -90,7 +93,9 @@ Bc error_bc(const sr_mg *m, int l, int zeroBC);
 // red-black sweeps start without a ghost fill
 void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
                   hipEvent_t ea = nullptr, hipEvent_t eb = nullptr, bool faces_ready = false);
-void residual_level(sr_mg *m, int l, int color = -1);
+// images: with color 0 and image residual-pad rules the kernel writes the face and edge ghosts the
+// red-black restriction reads instead of a separate fill
+void residual_level(sr_mg *m, int l, int color = -1, bool images = false);
 void coarsest_level(sr_mg *m, int l, int zeroBC);
 void vcycle_fp64(sr_mg *m);
 void vcycle_fp32(sr_mg *m);
