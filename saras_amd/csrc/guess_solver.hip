@@ -453,8 +453,17 @@ __global__ void __launch_bounds__(256) k_wsum_pc(FineDims f, PlaneChunks pc, con
     if (r < pc.n1 * pc.n2 && j >= 0 && j < f.ny && k >= 0 && k < f.nz) {
         const double wjk = wy[j] * wz[k];
         const long ps = (long)pc.n1 * pc.n2;
-#pragma unroll 4
-        for (unsigned ip = 1 + g; ip <= (unsigned)f.nx; ip += pc.groups) v[0] += wx[ip - 1] * wjk * rhs[ip * ps + r];
+        // planes 1 + g + q groups in batches of eight loads issued before any is added (plane order)
+        const unsigned nq = 1 + g <= (unsigned)f.nx ? ((unsigned)f.nx - 1 - g) / pc.groups + 1 : 0;
+        unsigned q = 0;
+        for (; q + 8 <= nq; q += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = rhs[(1 + g + (q + u) * pc.groups) * ps + r];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[0] += wx[g + (q + u) * pc.groups] * wjk * x[u];
+        }
+        for (; q < nq; q++) v[0] += wx[g + q * pc.groups] * wjk * rhs[(1 + g + q * pc.groups) * ps + r];
     }
     block_sums<1>(v, part);
 }

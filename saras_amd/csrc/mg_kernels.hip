@@ -650,20 +650,37 @@ __global__ void __launch_bounds__(256) k_f_terms(Lay L, Geo G, const T *__restri
 }
 
 // Weighted core sum q0 = sum w_x(i) w_y(j) w_z(k) ref of a reference padded array (the projection's
-// left-null-vector weight), read in place: work item li = (core plane p, 256-entry chunk of the
-// plane), the plane index uniform per block; per-block partials {q0, 0, 0, 0}.
-__global__ void __launch_bounds__(256) k_wsum_ref(Lay L, unsigned bpp, float inv2, const double *__restrict__ ref,
-                                                  const double *__restrict__ wx, const double *__restrict__ wy,
-                                                  const double *__restrict__ wz, double *__restrict__ part) {
-    const unsigned n1 = (unsigned)L.ny + 2, n2 = (unsigned)L.nz + 2, total = (unsigned)L.nx * bpp;
+// left-null-vector weight), read in place.  Work item w covers in-plane entries [256 (w % bpp),
+// +256) of the core planes 1 + w / bpp, 1 + w / bpp + groups, ...: each thread's (j, k) and its
+// core test are fixed, so the loop over planes is division-free and its loads overlap.  Per-block
+// partials {q0, 0, 0, 0}.
+__global__ void __launch_bounds__(256) k_wsum_ref(Lay L, unsigned bpp, unsigned groups, float inv2,
+                                                  const double *__restrict__ ref, const double *__restrict__ wx,
+                                                  const double *__restrict__ wy, const double *__restrict__ wz,
+                                                  double *__restrict__ part) {
+    const unsigned n1 = (unsigned)L.ny + 2, n2 = (unsigned)L.nz + 2;
     double s0 = 0.0;
-#pragma unroll 4
-    for (unsigned li = blockIdx.x; li < total; li += gridDim.x) {
-        const unsigned p = li / bpp;
-        const unsigned r = (li - p * bpp) * 256u + threadIdx.x;
+    // one pass of the outer loop unless a plane has more than REDUCE_BLOCKS chunks
+    for (unsigned w = blockIdx.x; w < bpp * groups; w += gridDim.x) {
+        const unsigned chunk = w % bpp, g = w / bpp;
+        const unsigned r = chunk * 256u + threadIdx.x;
         const unsigned jj = fdivu(r, n2, inv2), kk = r - jj * n2;
-        if (r < n1 * n2 && jj >= 1 && jj <= (unsigned)L.ny && kk >= 1 && kk <= (unsigned)L.nz)
-            s0 += wx[p] * wy[jj - 1] * wz[kk - 1] * ref[(long)(p + 1) * (n1 * n2) + r];
+        if (r < n1 * n2 && jj >= 1 && jj <= (unsigned)L.ny && kk >= 1 && kk <= (unsigned)L.nz) {
+            const double wjk = wy[jj - 1] * wz[kk - 1];
+            const long ps = (long)n1 * n2;
+            // the planes 1 + g + q groups, q < nq, in batches of eight loads issued before any is
+            // added (the sum runs in plane order, as a plain loop would)
+            const unsigned nq = 1 + g <= (unsigned)L.nx ? ((unsigned)L.nx - 1 - g) / groups + 1 : 0;
+            unsigned q = 0;
+            for (; q + 8 <= nq; q += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = ref[(1 + g + (q + u) * groups) * ps + r];
+#pragma unroll
+                for (int u = 0; u < 8; u++) s0 += wx[g + (q + u) * groups] * wjk * v[u];
+            }
+            for (; q < nq; q++) s0 += wx[g + q * groups] * wjk * ref[(1 + g + q * groups) * ps + r];
+        }
     }
     block_reduce4(s0, 0.0, 0.0, 0.0, part);
 }
@@ -1804,11 +1821,15 @@ void launch_f_terms(const Lay &L, const T *f, double *part, double *fterms, hipS
 
 void launch_wsum_ref(const Lay &L, const double *ref, const double *wx, const double *wy, const double *wz,
                      double *part, double *out, hipStream_t s) {
+    // in-plane entries below 2^24 (fdivu); at most REDUCE_BLOCKS partials (the rows of part)
     const unsigned n2 = (unsigned)L.nz + 2;
     const unsigned bpp = ((unsigned)(L.ny + 2) * n2 + 255u) / 256u;
-    const unsigned total = (unsigned)L.nx * bpp;
-    const unsigned nb = total < REDUCE_BLOCKS ? total : REDUCE_BLOCKS;
-    hipLaunchKernelGGL(k_wsum_ref, dim3(nb), dim3(256), 0, s, L, bpp, 1.0f / (float)n2, ref, wx, wy, wz, part);
+    unsigned groups = REDUCE_BLOCKS / bpp;
+    if (groups > (unsigned)L.nx) groups = (unsigned)L.nx;
+    if (groups < 1u) groups = 1u;
+    const unsigned nb = bpp * groups < REDUCE_BLOCKS ? bpp * groups : REDUCE_BLOCKS;
+    hipLaunchKernelGGL(k_wsum_ref, dim3(nb), dim3(256), 0, s, L, bpp, groups, 1.0f / (float)n2, ref, wx, wy, wz,
+                       part);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, part, (int)nb, out);
 }
 
