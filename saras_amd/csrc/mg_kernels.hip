@@ -1502,10 +1502,26 @@ template <typename T>
 __device__ inline void blk_direct(const Lay &L, T *u, const T *f, const T *__restrict__ pt, int nc) {
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         double acc = 0.0;
-        int j = 0;
-        for (int a = 1; a <= L.nx; a++)
-            for (int b = 1; b <= L.ny; b++)
-                for (int c = 1; c <= L.nz; c++, j++) acc += (double)pt[(long)j * nc + i] * (double)f[lay_idx(L, a, b, c)];
+        // eight entries of the operator's column loaded before any is used (they come from global
+        // memory); the products are added in node order
+        for (int j0 = 0; j0 < nc; j0 += 8) {
+            double p[8];
+            T fv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int j = j0 + q;
+                p[q] = j < nc ? (double)pt[(long)j * nc + i] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int j = j0 + q < nc ? j0 + q : nc - 1;
+                const int c = j % L.nz, r = j / L.nz;
+                fv[q] = f[lay_idx(L, 1 + r / L.ny, 1 + r % L.ny, 1 + c)];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (j0 + q < nc) acc += p[q] * (double)fv[q];
+        }
         int k = i % L.nz, r = i / L.nz;
         u[lay_idx(L, 1 + r / L.ny, 1 + r % L.ny, 1 + k)] = (T)acc;
     }
@@ -1519,20 +1535,41 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
     T *base = (T *)smem;
     __shared__ double red[16];
     const int nl = D.last - D.first + 1;
-    // coefficients and the right-hand side of the first level into LDS; u(first) starts at zero
-    for (int q = 0; q < nl; q++) {
-        const Lay &L = D.L[q];
-        int nn[3] = {L.nx + 2, L.ny + 2, L.nz + 2};
-        for (int a = 0; a < 3; a++) {
-            const T *src = (const T *)D.cg[q][a];
-            T *dst = base + D.off_c[q][a];
-            for (int t = threadIdx.x; t < 4 * nn[a]; t += blockDim.x) dst[t] = src[t];
-        }
+    // coefficients and the right-hand side of the first level into LDS; u(first) starts at zero.
+    // The global loads are issued in batches before their LDS stores (each is a round trip to L2
+    // or memory; the copies of one workgroup would otherwise run one after another).
+    {
+        T cv[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int cnt = q < nl ? 4 * ((a == 0 ? D.L[q].nx : a == 1 ? D.L[q].ny : D.L[q].nz) + 2) : 0;
+                cv[q][a] = (int)threadIdx.x < cnt ? ((const T *)D.cg[q][a])[threadIdx.x] : T(0);
+            }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int cnt = q < nl ? 4 * ((a == 0 ? D.L[q].nx : a == 1 ? D.L[q].ny : D.L[q].nz) + 2) : 0;
+                if ((int)threadIdx.x < cnt) base[D.off_c[q][a] + threadIdx.x] = cv[q][a];
+                // arrays longer than the workgroup (not met by the coarse levels) copy the rest
+                for (int t = (int)threadIdx.x + (int)blockDim.x; t < cnt; t += blockDim.x)
+                    base[D.off_c[q][a] + t] = ((const T *)D.cg[q][a])[t];
+            }
     }
     {
-        long n = lay_size(D.L[0]);
+        const long n = lay_size(D.L[0]);
         T *f = base + D.off_f[0];
-        for (long t = threadIdx.x; t < n; t += blockDim.x) f[t] = f_first[t];
+        const long step = blockDim.x;
+        for (long t0 = threadIdx.x; t0 < n; t0 += 8 * step) {
+            T v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = t0 + q * step < n ? f_first[t0 + q * step] : T(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (t0 + q * step < n) f[t0 + q * step] = v[q];
+        }
     }
     __syncthreads();
 #define CX(q) (base + D.off_c[q][0])
