@@ -1237,6 +1237,67 @@ __device__ inline void blk_restrict_taps(const Lay &F, const T *uf, const T *ff,
     }
 }
 
+// The red-black forms of the two restrictions above: after a red-black sweep that ended on colour 1
+// the colour-1 residual is zero, and of the 27 points only the coincident node and the 12 edge
+// points are colour 0 (k_restrict<T, true>): 13 residual evaluations per coarse node, weighted in
+// k_restrict<T, true>'s order.  kRbTaps lists the 12 edge offsets in that order, then the centre.
+__constant__ signed char kRbTaps[13][3] = {{1, 1, 0},  {1, -1, 0}, {-1, -1, 0}, {-1, 1, 0}, {0, 1, 1},
+                                           {0, -1, 1}, {0, -1, -1}, {0, 1, -1}, {1, 0, 1},  {1, 0, -1},
+                                           {-1, 0, -1}, {-1, 0, 1}, {0, 0, 0}};
+constexpr int kRbTapNodes = kTapScratch / 13;
+
+template <typename T, typename R>
+__device__ inline T rb_weight13(R &&res) {
+    T edgePoints = (res(0) + res(1) + res(2) + res(3) + res(4) + res(5) + res(6) + res(7) + res(8) + res(9) +
+                    res(10) + res(11)) *
+                   T(0.03125);
+    return edgePoints + res(12) * T(0.125);
+}
+
+template <typename T>
+__device__ inline void blk_restrict_rb(const Lay &F, const T *uf, const T *ff, const T *cx, const T *cy, const T *cz,
+                                       const Bc &tb, const Lay &Cl, T *fc) {
+    int n = Cl.nx * Cl.ny * Cl.nz;
+    const Div3 dv = div3(Cl.ny, Cl.nz);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        int i, j, k;
+        split3(dv, t, i, j, k);
+        int fi = 2 * i + 1, fj = 2 * j + 1, fk = 2 * k + 1;
+        fc[lay_idx(Cl, i + 1, j + 1, k + 1)] = rb_weight13<T>([&](int q) {
+            return blk_res_src<T>(F, uf, ff, cx, cy, cz, tb, fi + kRbTaps[q][0], fj + kRbTaps[q][1],
+                                  fk + kRbTaps[q][2]);
+        });
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ inline void blk_restrict_rb_taps(const Lay &F, const T *uf, const T *ff, const T *cx, const T *cy,
+                                            const T *cz, const Bc &tb, const Lay &Cl, T *fc, T *scr) {
+    const int n = Cl.nx * Cl.ny * Cl.nz;
+    const Div3 dv = div3(Cl.ny, Cl.nz);
+    const int t = threadIdx.x;
+    for (int first = 0; first < n; first += kRbTapNodes) {
+        if (t < 13 * kRbTapNodes) {
+            int node = first + t / 13, tap = t % 13;
+            if (node < n) {
+                int i, j, k;
+                split3(dv, node, i, j, k);
+                scr[t] = blk_res_src<T>(F, uf, ff, cx, cy, cz, tb, 2 * i + 1 + kRbTaps[tap][0],
+                                        2 * j + 1 + kRbTaps[tap][1], 2 * k + 1 + kRbTaps[tap][2]);
+            }
+        }
+        __syncthreads();
+        if (t < kRbTapNodes && first + t < n) {
+            const T *R = scr + 13 * t;
+            int i, j, k;
+            split3(dv, first + t, i, j, k);
+            fc[lay_idx(Cl, i + 1, j + 1, k + 1)] = rb_weight13<T>([&](int q) { return R[q]; });
+        }
+        __syncthreads();
+    }
+}
+
 // uf += P e on the core nodes of the fine level (trilinear, poisson_d3.cc:301-388).  The items
 // t, t + blockDim, t + 2 blockDim of a thread update distinct nodes, so their loads are all issued
 // before the first update is stored.  With `images` every updated node also rewrites its face-ghost
@@ -1244,18 +1305,30 @@ __device__ inline void blk_restrict_taps(const Lay &F, const T *uf, const T *ff,
 constexpr int kBlkBatch = 3;
 
 template <typename T>
-__device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const T *e, const Bc &bc, bool images) {
-    const int n = F.nx * F.ny * F.nz;
-    const Div3 dv = div3(F.ny, F.nz);
+__device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const T *e, const Bc &bc, bool images,
+                                       int color = -1) {
+    // color 1: the colour-1 nodes only (slots (i, j, h), kp = 2h + parity), the post-smoothing's first
+    // colour-0 half sweep overwrites the colour-0 nodes without reading them
+    const int kc = color == 1 ? (F.nz + 2) / 2 : F.nz;
+    const int n = F.nx * F.ny * kc;
+    const Div3 dv = div3(F.ny, kc);
     for (int t0 = threadIdx.x; t0 < n; t0 += kBlkBatch * blockDim.x) {
         T val[kBlkBatch];
         int pi[kBlkBatch], pj[kBlkBatch], pk[kBlkBatch];
+        bool ok[kBlkBatch];
 #pragma unroll
         for (int b = 0; b < kBlkBatch; b++) {
             const int t = t0 + b * (int)blockDim.x;
+            ok[b] = false;
             if (t >= n) continue;
             int i, j, k;
             split3(dv, t, i, j, k);
+            if (color == 1) {
+                const int kp = 2 * k + ((i + j) & 1);
+                if (kp < 1 || kp > F.nz) continue;
+                k = kp - 1;
+            }
+            ok[b] = true;
             int io = i & 1, jo = j & 1, ko = k & 1;
             int ci = i / 2 + 1, cj = j / 2 + 1, ck = k / 2 + 1;
 #define EE(a, b2, q) e[lay_idx(Cl, ci + (a), cj + (b2), ck + (q))]
@@ -1285,7 +1358,7 @@ __device__ inline void blk_prolong_add(const Lay &F, T *uf, const Lay &Cl, const
         }
 #pragma unroll
         for (int b = 0; b < kBlkBatch; b++) {
-            if (t0 + b * (int)blockDim.x >= n) continue;
+            if (!ok[b]) continue;
             uf[lay_idx(F, pi[b], pj[b], pk[b])] = val[b];
             if (images) write_images(F, uf, pi[b], pj[b], pk[b], val[b], bc);
         }
@@ -1535,6 +1608,11 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
     T *base = (T *)smem;
     __shared__ double red[16];
     const int nl = D.last - D.first + 1;
+    // the coarsest level's dense operator (exact coarsest solve), one entry per thread, loaded now and
+    // placed in the restriction scratch for the coarsest phase, where it fits
+    const int nc2 = D.pinv ? D.ncd * D.ncd : 0;
+    const bool pt_lds = D.off_s >= 0 && nc2 > 0 && nc2 <= kTapScratch;
+    const T ptv = pt_lds && (int)threadIdx.x < nc2 ? ((const T *)D.pinv)[threadIdx.x] : T(0);
     // coefficients and the right-hand side of the first level into LDS; u(first) starts at zero.
     // The global loads are issued in batches before their LDS stores (each is a round trip to L2
     // or memory; the copies of one workgroup would otherwise run one after another).
@@ -1588,10 +1666,17 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
             blk_zero<T>(L, UU(q));
         }
         const Lay &C = D.L[q + 1];
-        if (D.off_s >= 0 && C.nx * C.ny * C.nz <= D.tap_max)
+        if (D.pre > 0) {
+            // the pre-smoothing ended on colour 1: only the colour-0 residual points are restricted
+            if (D.off_s >= 0 && C.nx * C.ny * C.nz <= 4 * kRbTapNodes)
+                blk_restrict_rb_taps<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, C, FF(q + 1), base + D.off_s);
+            else
+                blk_restrict_rb<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, C, FF(q + 1));
+        } else if (D.off_s >= 0 && C.nx * C.ny * C.nz <= D.tap_max) {
             blk_restrict_taps<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, C, FF(q + 1), base + D.off_s);
-        else
+        } else {
             blk_restrict<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), D.tbc, C, FF(q + 1));
+        }
     }
     // coarsest solve (poisson_d3.cc:190-247 with the red-black sweep)
     {
@@ -1599,7 +1684,13 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
         const Lay &L = D.L[q];
         blk_zero<T>(L, UU(q));
         if (D.pinv) {
-            blk_direct<T>(L, UU(q), FF(q), (const T *)D.pinv, D.ncd);
+            const T *pt = (const T *)D.pinv;
+            if (pt_lds) {
+                if ((int)threadIdx.x < nc2) base[D.off_s + threadIdx.x] = ptv;
+                __syncthreads();
+                pt = base + D.off_s;
+            }
+            blk_direct<T>(L, UU(q), FF(q), pt, D.ncd);
             lds_fill_ghosts(L, UU(q), D.bc);
             if (threadIdx.x == 0) iters[0] = 0;
         } else {
@@ -1632,7 +1723,7 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(CoarseDesc D, T *__restri
     const bool images = D.bc.rule[0] <= G_ANTI && D.bc.rule[1] <= G_ANTI && D.bc.rule[2] <= G_ANTI;
     for (int q = nl - 2; q >= 0; q--) {
         const Lay &L = D.L[q];
-        blk_prolong_add<T>(L, UU(q), D.L[q + 1], UU(q + 1), D.bc, images);
+        blk_prolong_add<T>(L, UU(q), D.L[q + 1], UU(q + 1), D.bc, images, D.post > 0 ? 1 : -1);
         if (!images) lds_fill_ghosts(L, UU(q), D.bc);
         for (int s = 0; s < D.post; s++) {
             blk_sweep<T>(L, UU(q), FF(q), CX(q), CY(q), CZ(q), 0, D.bc, false);
