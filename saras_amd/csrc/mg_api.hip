@@ -478,7 +478,7 @@ void sr::mark(sr_mg *m, hipEvent_t e) {
 // smooth(): ghosts, then `count` red-black sweeps (each updated node refreshes its face ghosts);
 // exact = 1 adds the closing imposeBC of poisson_d3.cc:181 (edge and corner ghosts included).
 void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact, hipEvent_t ea,
-                      hipEvent_t eb, bool faces_ready) {
+                      hipEvent_t eb, bool faces_ready, bool first_done) {
     Level &V = m->lv[l];
     Bc bc = pressure_bc(m, l, zeroBC);
     hipStream_t s = m->stream;
@@ -490,7 +490,9 @@ void sr::smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, b
         if (zero_start && count == 0) launch_fill<double>(V.u, lay_size(V.L), 0.0, s);
         if (ea) mark(m, ea);
         for (int n = 0; n < count; n++) {
-            launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 0, bc, zero_start && n == 0, s);
+            // first_done: the first (zero-start) colour-0 half sweep was done by the restriction
+            if (!(first_done && n == 0))
+                launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 0, bc, zero_start && n == 0, s);
             launch_rb_sweep<double>(V.L, V.u, V.f, V.c64[0], V.c64[1], V.c64[2], 1, bc, false, s);
         }
         if (eb) mark(m, eb);
@@ -551,7 +553,12 @@ void sr::vcycle_fp64(sr_mg *m) {
     const bool red0 = d.gsSmooth && d.preSmooth > 0;
     for (int l = 0; l < bottom; l++) {
         residual_level(m, l, red0 ? 0 : -1, l > 0);
-        launch_restrict<double>(m->lv[l].L, m->lv[l + 1].L, m->lv[l].t, m->lv[l + 1].f, m->stream, red0);
+        // a level smoothed from zero by Gauss-Seidel starts with a colour-0 half sweep that reads no
+        // u: the restriction that makes its right-hand side does it (ZeroStart)
+        const bool fuse = d.gsSmooth && d.preSmooth > 0 && l + 1 != Lc && l + 1 != D;
+        Level &Nx = m->lv[l + 1];
+        const ZeroStart<double> zs{Nx.u, Nx.c64[0], Nx.c64[1], Nx.c64[2], pressure_bc(m, l + 1, 1)};
+        launch_restrict<double>(m->lv[l].L, Nx.L, m->lv[l].t, Nx.f, m->stream, red0, fuse ? &zs : nullptr);
         if (l + 1 == Lc) {
             launch_coarse_cycle<double>(m->cdesc64, m->clds64, m->lv[Lc].u, m->lv[Lc].f, m->diters, m->stream);
         } else if (l + 1 == D) {
@@ -559,7 +566,7 @@ void sr::vcycle_fp64(sr_mg *m) {
             if (d.coarseProject && singular(m)) project_coarsest(m);
             coarsest_level(m, D, 1);
         } else if (d.gsSmooth) {
-            smooth_level(m, l + 1, d.preSmooth, 1, true, false);
+            smooth_level(m, l + 1, d.preSmooth, 1, true, false, nullptr, nullptr, false, fuse);
         } else {
             launch_fill<double>(m->lv[l + 1].u, lay_size(m->lv[l + 1].L), 0.0, m->stream);
             smooth_level(m, l + 1, d.preSmooth, 1, false, false);
@@ -591,7 +598,7 @@ Bc sr::error_bc(const sr_mg *m, int l, int zeroBC) {
 }
 
 static void smooth32(sr_mg *m, int l, int count, int zeroBC, bool zero_start, hipEvent_t ea = nullptr,
-                     hipEvent_t eb = nullptr, bool faces_ready = false) {
+                     hipEvent_t eb = nullptr, bool faces_ready = false, bool first_done = false) {
     Level &V = m->lv[l];
     Bc bc = error_bc(m, l, zeroBC);
     bool wall = bc.rule[0] == G_WALL || bc.rule[1] == G_WALL || bc.rule[2] == G_WALL;
@@ -605,7 +612,9 @@ static void smooth32(sr_mg *m, int l, int count, int zeroBC, bool zero_start, hi
     }
     if (ea) mark(m, ea);
     for (int n = 0; n < count; n++) {
-        launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], 0, bc, zs && n == 0, s);
+        // first_done: the first (zero-start) colour-0 half sweep was done by the restriction
+        if (!(first_done && n == 0))
+            launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], 0, bc, zs && n == 0, s);
         launch_rb_sweep<float>(V.L, V.u32, V.f32, V.c32[0], V.c32[1], V.c32[2], 1, bc, false, s);
     }
     if (eb) mark(m, eb);
@@ -628,7 +637,13 @@ void sr::vcycle_fp32(sr_mg *m) {
         launch_residual<float, float>(V.L, V.u32, V.f32, V.t32, V.c32[0], V.c32[1], V.c32[2], s, red0 ? 0 : -1,
                                       img ? &tb : nullptr);
         if (!img) launch_fill_ghosts<float>(V.L, V.t32, tb, s);
-        launch_restrict<float>(V.L, m->lv[l + 1].L, V.t32, m->lv[l + 1].f32, s, red0);
+        // the next level's zero-start colour-0 half sweep done by its restriction (as in the FP64 cycle)
+        Level &Nx = m->lv[l + 1];
+        const Bc nb = error_bc(m, l + 1, 1);
+        const bool nwall = nb.rule[0] == G_WALL || nb.rule[1] == G_WALL || nb.rule[2] == G_WALL;
+        const bool fuse = m->pre32 > 0 && !nwall && l + 1 != Lc && l + 1 != D;
+        const ZeroStart<float> zs{Nx.u32, Nx.c32[0], Nx.c32[1], Nx.c32[2], nb};
+        launch_restrict<float>(V.L, Nx.L, V.t32, Nx.f32, s, red0, fuse ? &zs : nullptr);
         if (l + 1 == Lc) {
             launch_coarse_cycle<float>(m->cdesc32, m->clds32, m->lv[Lc].u32, m->lv[Lc].f32, m->diters, s);
         } else if (l + 1 == D) {
@@ -643,7 +658,7 @@ void sr::vcycle_fp32(sr_mg *m) {
             launch_coarsest<float>(Cv.L, Cv.u32, Cv.f32, Cv.c32[0], Cv.c32[1], Cv.c32[2], error_bc(m, D, 1),
                                    d.mgTolerance, m->maxCount, m->diters, s);
         } else {
-            smooth32(m, l + 1, m->pre32, 1, true);
+            smooth32(m, l + 1, m->pre32, 1, true, nullptr, nullptr, false, fuse);
         }
     }
     // colour 0 is swept first by the post-smoothing: its correction would be overwritten unread

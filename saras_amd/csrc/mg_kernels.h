@@ -19,8 +19,16 @@ void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, c
                      hipStream_t s, int color = -1, const Bc *img = nullptr);
 // red0: t is zero on colour 1 and only its colour-0 entries are read (residual after a red-black
 // sweep ending on colour 1, computed with color = 0)
+// zero-start colour-0 half sweep of the coarse level fused into its restriction (ZS in k_restrict)
 template <typename T>
-void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0 = false);
+struct ZeroStart {
+    T *u;
+    const T *cx, *cy, *cz;
+    Bc bc;
+};
+template <typename T>
+void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0 = false,
+                     const ZeroStart<T> *zs = nullptr);
 template <typename T, typename TF>
 // color >= 0: that colour's nodes only (the up leg before red-black smoothing that sweeps the other
 // colour first); -1: every core node.  img: the updated nodes also write their face-ghost images by *img

@@ -316,8 +316,12 @@ __global__ void __launch_bounds__(256) k_residual(Lay L, Geo G, const T *__restr
 // which leaves every colour-1 equation satisfied): the coincident node and the 12 edge points are
 // colour 0 (even offset sums), the 6 face and 8 vertex points colour 1, so only the former are read.
 // ------------------------------------------------------------------------------------------------
-template <typename T, bool RED0>
-__global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T *__restrict__ t, T *__restrict__ fc) {
+// ZS: the coarse level's first smoothing step, a zero-start colour-0 half sweep (u = -f / diagonal,
+// no u read), is done here for the colour-0 nodes from the value just restricted, with its
+// face-ghost images (the rest of the level's smoothing then starts with the colour-1 half sweep).
+template <typename T, bool RED0, bool ZS>
+__global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T *__restrict__ t, T *__restrict__ fc,
+                                                  ZeroStart<T> zs) {
     unsigned ipu, rr;
     if (!geo_slot(G, xcd_block(), ipu, rr)) return;
     int ip = (int)ipu;
@@ -335,7 +339,13 @@ __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T 
         T edgePoints = (TT(1, 1, 0) + TT(1, -1, 0) + TT(-1, -1, 0) + TT(-1, 1, 0) + TT(0, 1, 1) + TT(0, -1, 1) +
                         TT(0, -1, -1) + TT(0, 1, -1) + TT(1, 0, 1) + TT(1, 0, -1) + TT(-1, 0, -1) + TT(-1, 0, 1)) *
                        T(0.03125);
-        fc[lay_row(Cl, ip, jp, c) + h] = edgePoints + TT(0, 0, 0) * T(0.125);
+        const T val = edgePoints + TT(0, 0, 0) * T(0.125);
+        fc[lay_row(Cl, ip, jp, c) + h] = val;
+        if (ZS && c == 0) {
+            const T v = -val / diag3<T>(zs.cx + 4 * ip, zs.cy + 4 * jp, zs.cz + 4 * kp);
+            zs.u[lay_row(Cl, ip, jp, 0) + h] = v;
+            write_images(Cl, zs.u, ip, jp, kp, v, zs.bc);
+        }
         return;
     }
     T facePoints = (TT(1, 0, 0) + TT(-1, 0, 0) + TT(0, 1, 0) + TT(0, -1, 0) + TT(0, 0, 1) + TT(0, 0, -1)) * T(0.0625);
@@ -345,7 +355,13 @@ __global__ void __launch_bounds__(256) k_restrict(Lay F, Lay Cl, Geo G, const T 
     T vertPoints = (TT(1, 1, 1) + TT(1, 1, -1) + TT(1, -1, 1) + TT(-1, 1, 1) + TT(1, -1, -1) + TT(-1, 1, -1) +
                     TT(-1, -1, 1) + TT(-1, -1, -1)) *
                    T(0.015625);
-    fc[lay_row(Cl, ip, jp, c) + h] = facePoints + edgePoints + vertPoints + TT(0, 0, 0) * T(0.125);
+    const T val = facePoints + edgePoints + vertPoints + TT(0, 0, 0) * T(0.125);
+    fc[lay_row(Cl, ip, jp, c) + h] = val;
+    if (ZS && c == 0) {
+        const T v = -val / diag3<T>(zs.cx + 4 * ip, zs.cy + 4 * jp, zs.cz + 4 * kp);
+        zs.u[lay_row(Cl, ip, jp, 0) + h] = v;
+        write_images(Cl, zs.u, ip, jp, kp, v, zs.bc);
+    }
 #undef TT
 }
 
@@ -1804,12 +1820,18 @@ void launch_residual(const Lay &L, const T *u, const T *f, TO *t, const T *cx, c
 }
 
 template <typename T>
-void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0) {
+void launch_restrict(const Lay &F, const Lay &Cl, const T *t, T *fc, hipStream_t s, bool red0,
+                     const ZeroStart<T> *zs) {
     Geo G = geo_core(Cl);
-    if (red0)
-        hipLaunchKernelGGL((k_restrict<T, true>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc);
+    const ZeroStart<T> z = zs ? *zs : ZeroStart<T>{};
+    if (red0 && zs)
+        hipLaunchKernelGGL((k_restrict<T, true, true>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc, z);
+    else if (red0)
+        hipLaunchKernelGGL((k_restrict<T, true, false>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc, z);
+    else if (zs)
+        hipLaunchKernelGGL((k_restrict<T, false, true>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc, z);
     else
-        hipLaunchKernelGGL((k_restrict<T, false>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc);
+        hipLaunchKernelGGL((k_restrict<T, false, false>), dim3(G.total_blocks), dim3(256), 0, s, F, Cl, G, t, fc, z);
 }
 
 template <typename T, typename TF>
@@ -2022,7 +2044,8 @@ int coarse_cycle_attr(size_t lds) {
     template void launch_fill_ghosts<T>(const Lay &, T *, const Bc &, hipStream_t);                        \
     template void launch_rb_sweep<T>(const Lay &, T *, const T *, const T *, const T *, const T *, int,     \
                                      const Bc &, bool, hipStream_t);                                        \
-    template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t, bool);         \
+    template void launch_restrict<T>(const Lay &, const Lay &, const T *, T *, hipStream_t, bool,          \
+                                     const ZeroStart<T> *);                                                 \
     template void launch_fill<T>(T *, long, T, hipStream_t);                                               \
     template void launch_add_scalar<T>(const Lay &, T *, const double *, double, int, hipStream_t);        \
     template void launch_norm_res<T>(const Lay &, const T *, const T *, const T *, const T *, const T *,   \

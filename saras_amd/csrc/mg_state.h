@@ -91,8 +91,10 @@ Bc tmp_bc(const sr_mg *m);
 Bc error_bc(const sr_mg *m, int l, int zeroBC);
 // faces_ready: the level's face-ghost images are current (written by the prolongation), so the
 // red-black sweeps start without a ghost fill
+// first_done: the first (zero-start) colour-0 half sweep was done by the level's restriction
 void smooth_level(sr_mg *m, int l, int count, int zeroBC, bool zero_start, bool exact,
-                  hipEvent_t ea = nullptr, hipEvent_t eb = nullptr, bool faces_ready = false);
+                  hipEvent_t ea = nullptr, hipEvent_t eb = nullptr, bool faces_ready = false,
+                  bool first_done = false);
 // images: with color 0 and image residual-pad rules the kernel writes the face and edge ghosts the
 // red-black restriction reads instead of a separate fill
 void residual_level(sr_mg *m, int l, int color = -1, bool images = false);
